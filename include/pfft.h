@@ -1,0 +1,175 @@
+/*
+ * pfft.h -- C ABI of the MI355X-native PencilFFTs transform pipeline.
+ *
+ * One library, libpencilffts_hip.so, replaces the execution layer of
+ * PencilFFTs.jl v0.15.3 for the single-GPU case: the per-dimension FFTW calls
+ * (src/operations.jl:250) and the PencilArrays transposes between them
+ * (src/operations.jl:239-240) become one stream-ordered sequence of fused HIP
+ * passes.  Each entry point below names the reference interface it stands in
+ * for.  All functions return an int32 status (PFFT_OK == 0) and never throw;
+ * the message of the last failure on the calling thread is returned by
+ * pfft_last_error().
+ *
+ * Array layout follows Julia: column-major, memory dims listed fastest first.
+ * The input of a plan is laid out in logical order (x, y, z, ..., extra...),
+ * the output in the memory order of the last pencil of the reference's chain
+ * (reversed dims (z, y, x, extra...) for permute_dims = true), exactly as
+ * PencilFFTs' allocate_output returns it (src/allocate.jl:106-110).
+ */
+#ifndef PFFT_H
+#define PFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  PFFT_OK = 0,
+  PFFT_ERR_ARGUMENT = 1,    /* the reference throws ArgumentError here */
+  PFFT_ERR_UNSUPPORTED = 2, /* valid in the reference, not in this build */
+  PFFT_ERR_DEVICE = 3,      /* a HIP runtime call failed */
+  PFFT_ERR_ALLOC = 4        /* host or device allocation failed */
+};
+
+/* ---- transform kinds (one per dimension) -------------------------------
+ * Mirrors the transform types of src/Transforms/{c2c,r2c,r2r,no_transform}.jl.
+ * R2R kinds are PFFT_R2R + the FFTW r2r kind code (FFTW_R2HC = 0 ... FFTW_RODFT11
+ * = 10); R2HC and HC2R are rejected like the reference does (r2r.jl:40-46).
+ * OR-ing PFFT_INPLACE gives the in-place variant (FFT!, RFFT!, ...).        */
+enum {
+  PFFT_NOTRANSFORM = 0,
+  PFFT_FFT = 1,
+  PFFT_BFFT = 2,
+  PFFT_RFFT = 3,
+  PFFT_BRFFT_EVEN = 4,
+  PFFT_BRFFT_ODD = 5,
+  PFFT_R2R = 16,
+  PFFT_INPLACE = 0x100
+};
+enum {
+  PFFT_R2HC = 0, PFFT_HC2R = 1, PFFT_DHT = 2,
+  PFFT_REDFT00 = 3, PFFT_REDFT01 = 4, PFFT_REDFT10 = 5, PFFT_REDFT11 = 6,
+  PFFT_RODFT00 = 7, PFFT_RODFT01 = 8, PFFT_RODFT10 = 9, PFFT_RODFT11 = 10
+};
+
+/* ---- element type codes ------------------------------------------------ */
+enum { PFFT_F32 = 0, PFFT_F64 = 1, PFFT_C32 = 2, PFFT_C64 = 3, PFFT_NOTHING = -1 };
+
+/* ---- execute directions (src/operations.jl:24-52) ---------------------- */
+enum {
+  PFFT_FORWARD = 0,          /* mul!(dst, p, src)          operations.jl:24-31 */
+  PFFT_BACKWARD = 1,         /* PencilFFTs.bmul!(...)      operations.jl:34-41 */
+  PFFT_BACKWARD_SCALED = 2   /* ldiv!(dst, p, src)         operations.jl:44-52 */
+};
+
+/* ---- plan flags -------------------------------------------------------- */
+enum {
+  PFFT_FLAG_GENERIC_ONLY = 1u << 0, /* use only the generic LDS kernels */
+  PFFT_FLAG_NO_AUTOTUNE = 1u << 1,  /* take the heuristic tile shapes */
+  PFFT_FLAG_MFMA_SPLIT16 = 1u << 2, /* radix-16 stages on MFMA, split-FP16 */
+  PFFT_FLAG_MFMA_BF16X3 = 1u << 3   /* radix-16 stages on MFMA, BF16x3 */
+};
+
+typedef struct pfft_plan pfft_plan;
+
+/* ---- library ----------------------------------------------------------- */
+const char* pfft_version(void);
+const char* pfft_last_error(void);
+
+/* ---- 1D transform metadata: src/Transforms/Transforms.jl public API ------
+ * length_output  c2c.jl:49, r2c.jl:74-75, r2r.jl:82, no_transform.jl:23
+ * binv           c2c.jl:58-61, r2c.jl:99-102, r2r.jl:103-131, no_transform.jl:21
+ * scale_factor   c2c.jl:71, r2c.jl:104-111, r2r.jl:96-101, no_transform.jl:25
+ *                (for one transformed dimension of length n_in; for BRFFT n_in
+ *                is the complex input length)
+ * eltype_input   c2c.jl:51, r2c.jl:80-81, r2r.jl:83 (PFFT_NOTHING = `nothing`)
+ * eltype_output  c2c.jl:50, r2c.jl:77-78, r2r.jl:84, Transforms.jl:263-264
+ * is_inplace     Transforms.jl:139-147 (tuple form: 1, 0, or -1 = `nothing`)
+ * expand_dims    Transforms.jl:285-291, r2c.jl:114-124
+ * transform_name Base.show of a transform (Transforms.jl:293-297, r2c.jl:64,
+ *                r2r.jl:75-76)                                              */
+int pfft_transform_check(int32_t kind);
+int pfft_transform_length_output(int32_t kind, int64_t n_in, int64_t* n_out);
+int pfft_transform_binv(int32_t kind, int64_t d, int32_t* kind_inv);
+int pfft_transform_scale_factor(int32_t kind, int64_t n_in, int64_t* scale);
+int pfft_transform_eltype_input(int32_t kind, int32_t real_type, int32_t* eltype);
+int pfft_transform_eltype_output(int32_t kind, int32_t eltype_in, int32_t* eltype);
+int pfft_transforms_is_inplace(int32_t n, const int32_t* kinds, int32_t* result);
+int pfft_transform_expand_dims(int32_t kind, int32_t n, int32_t* kinds_out);
+int pfft_transform_name(int32_t kind, char* buf, size_t buflen);
+
+/* ---- plans ---------------------------------------------------------------
+ * pfft_plan_create stands in for
+ *   PencilFFTPlan(dims_global, transforms, proc_dims, comm, real_type;
+ *                 extra_dims, permute_dims)               plans.jl:306-314
+ * and runs the PLAN layer: GlobalFFTParams (global_params.jl:50-62), input
+ * eltype inference (global_params.jl:81-95), the pencil/permutation chain
+ * (plans.jl:397-521), the per-dimension binv and scale factors
+ * (plans.jl:523-553) and their product (plans.jl:268).  proc_dims must
+ * multiply to one process (one GPU).  Creating a plan touches no GPU: device
+ * tables are built on the first execute (or by pfft_plan_prepare).          */
+int pfft_plan_create(pfft_plan** out, int32_t ndims, const int64_t* dims_logical,
+                     const int32_t* kinds, int32_t real_type, int32_t permute_dims,
+                     int32_t n_extra, const int64_t* extra_dims, uint32_t flags);
+int pfft_plan_destroy(pfft_plan* plan);
+int pfft_plan_prepare(pfft_plan* plan, void* hip_stream);
+
+/* Queries (plans.jl:555-591 accessors, allocate.jl shapes). */
+int pfft_plan_ndims(const pfft_plan* plan, int32_t* ndims, int32_t* n_extra);
+int pfft_plan_size_global(const pfft_plan* plan, int32_t which, int64_t* dims);
+int pfft_plan_mem_dims(const pfft_plan* plan, int32_t which, int64_t* dims);
+int pfft_plan_permutation(const pfft_plan* plan, int32_t which, int32_t* perm);
+int pfft_plan_eltypes(const pfft_plan* plan, int32_t* eltype_in, int32_t* eltype_out);
+int pfft_plan_is_inplace(const pfft_plan* plan, int32_t* inplace);
+int pfft_plan_scale_factor(const pfft_plan* plan, double* scale);
+int pfft_plan_kinds(const pfft_plan* plan, int32_t* kinds_fwd, int32_t* kinds_bwd);
+int pfft_plan_nbytes(const pfft_plan* plan, size_t* in_bytes, size_t* out_bytes,
+                     size_t* work_bytes);
+/* Per-stage pencil chain: memory permutation (1-based logical dims, fastest
+ * first) and logical global dims of the stage's input and output pencils,
+ * and the memory axis the 1D transform runs along (plans.jl:531-538).     */
+int pfft_plan_stage(const pfft_plan* plan, int32_t stage, int32_t* perm_in,
+                    int64_t* dims_in, int32_t* perm_out, int64_t* dims_out,
+                    int32_t* mem_axis);
+int pfft_plan_describe(const pfft_plan* plan, char* buf, size_t buflen);
+
+/* ---- execution -----------------------------------------------------------
+ * pfft_execute stands in for mul! / PencilFFTs.bmul! / ldiv! on a
+ * PencilFFTPlan (operations.jl:24-52): forward stages 1..N, backward stages
+ * N..1 with the binv transforms, and for PFFT_BACKWARD_SCALED the
+ * normalisation `dst .*= inv(scale_factor(p))` (operations.jl:50,64-66)
+ * fused into the last pass.  `in` and `out` are device pointers sized by
+ * pfft_plan_nbytes for the direction (forward: in = input array, out = output
+ * array; backward: in = output-shaped array, out = input-shaped array).
+ * `work` must hold work_bytes bytes, or be NULL to use a plan-owned buffer
+ * (like the reference's ibuf/obuf, plans.jl:233-234, which makes the plan
+ * non-reentrant).  Overlapping in/out ranges are rejected with the
+ * reference's "out-of-place plan applied to aliased data"
+ * (operations.jl:91-93).  All work is enqueued on `hip_stream` (NULL = the
+ * null stream); the call does not synchronise.                              */
+int pfft_execute(pfft_plan* plan, int32_t direction, const void* in, void* out,
+                 void* work, size_t work_bytes, void* hip_stream);
+
+/* Many arrays through one plan: the reference broadcasts mul!/ldiv!/bmul!
+ * over tuples or arrays of PencilArrays one at a time (operations.jl:139-152,
+ * with the length check of :130-137).  Here all n_arrays transforms are
+ * issued back to back on one stream with one shared workspace.              */
+int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
+                      const void* const* ins, void* const* outs, void* work,
+                      size_t work_bytes, void* hip_stream);
+
+/* Kernel timing of the last execute on a plan with profiling enabled: the
+ * mirror of the reference's TimerOutputs sections "FFT" per stage
+ * (operations.jl:250).  Enabled with pfft_plan_set_timing(plan, 1).        */
+int pfft_plan_set_timing(pfft_plan* plan, int32_t enable);
+int pfft_plan_stage_times(pfft_plan* plan, int32_t n, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFFT_H */

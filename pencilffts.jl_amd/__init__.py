@@ -1,0 +1,17 @@
+"""MI355X-native PencilFFTs transform pipeline (one GPU).
+
+Drop-in for PencilFFTs.jl's PencilFFTPlan with Transforms.FFT/BFFT/RFFT/
+BRFFT/R2R/NoTransform: plans are built by the C++ plan layer and executed by
+hand-written HIP kernels for gfx950 through the C ABI of include/pfft.h.
+"""
+from . import _lib
+from . import transforms as Transforms
+from ._lib import ArgumentError, DeviceError, PfftError, UnsupportedError
+from .plans import (Pencil, PencilArray, PencilFFTPlan, Val, allocate_input, allocate_output,
+                    bmul_, execute_many, ldiv_, mul_, scale_factor)
+
+__all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "Pencil", "allocate_input",
+           "allocate_output", "mul_", "ldiv_", "bmul_", "scale_factor", "Val",
+           "execute_many", "ArgumentError", "UnsupportedError", "DeviceError", "PfftError"]
+
+__version__ = _lib.lib.pfft_version().decode()

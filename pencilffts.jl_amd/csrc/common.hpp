@@ -1,0 +1,103 @@
+// Shared definitions of the transform pipeline: per-line operation kinds, the
+// pass descriptor handed to every kernel, and a small complex type usable on
+// host and device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PFFT_HD __host__ __device__ __forceinline__
+
+namespace pfft {
+
+constexpr int kMaxDims = 8;       // transformed dims + the flattened extra dim
+constexpr int kMaxStages = 24;    // radix stages of one line FFT
+constexpr int kBlockThreads = 256;
+
+// What one pass does to each line.  A pass transforms one logical dimension
+// of the array (the reference's `mul!(parent(v), r.fftw_plan, parent(u))`,
+// operations.jl:250) and writes it in the memory order of the next pencil
+// (the transpose of operations.jl:239-240, fused into the store).
+enum LineKind : int32_t {
+  L_COPY = 0,      // NoTransform / IdentityPlan (no_transform.jl:31)
+  L_C2C = 1,       // FFT / BFFT (sign from PassArgs::bwd)
+  L_R2C_HALF = 2,  // RFFT, even length: n/2-point complex FFT + split
+  L_R2C_FULL = 3,  // RFFT, odd length: n-point complex FFT of real data
+  L_C2R_HALF = 4,  // BRFFT{even}: Hermitian merge + n/2-point complex FFT
+  L_C2R_FULL = 5,  // BRFFT{odd}: Hermitian extension + n-point FFT
+  L_DCT1 = 6,      // REDFT00: even extension, 2(n-1)-point FFT
+  L_DCT2 = 7,      // REDFT10: Makhoul reorder, n-point FFT, post-twiddle
+  L_DCT3 = 8,      // REDFT01: pre-twiddle, n-point FFT, Makhoul un-reorder
+  L_DCT4 = 9,      // REDFT11: pre-twiddle, zero-padded 2n-point FFT, post-twiddle
+  L_DST1 = 10,     // RODFT00: odd extension, 2(n+1)-point FFT
+  L_DST2 = 11,     // RODFT10: DCT2 of (-1)^j x, reversed output
+  L_DST3 = 12,     // RODFT01: (-1)^k DCT3 of the reversed input
+  L_DST4 = 13,     // RODFT11: (-1)^k DCT4 of the reversed input
+  L_DHT = 14       // DHT: Re F - Im F
+};
+
+// Descriptor of one pass over the whole array (passed by value to kernels).
+//
+// The array is viewed as a set of lines along the transformed axis.  A line is
+// named by a multi-index m over `D` line dims; dim 0 is the tile axis, along
+// which `T` consecutive lines are processed by one workgroup so that the
+// transposed store writes T consecutive elements.  Element j of line m sits at
+//   in  + (sum_d m_d * lsin[d]  + j * sk_in)   (input elements)
+//   out + (sum_d m_d * lsout[d] + k * sk_out)  (output elements)
+struct PassArgs {
+  const void* in;
+  void* out;
+  const void* tw;      // M-point twiddles exp(-2 pi i k / M), k < M
+  const void* aux;     // per-kind pre/post twiddles (may be null)
+  const int32_t* radix_sched;  // unused by device code; kept for layout
+  int32_t kind;        // LineKind
+  int32_t bwd;         // 1: backward sign (conjugated twiddles)
+  int32_t in_cplx;     // input elements complex
+  int32_t out_cplx;    // output elements complex
+  int32_t r2r_cplx;    // R2R/COPY applied to complex data (re and im parts)
+  int32_t T;           // lines per tile
+  int32_t D;           // number of line dims
+  int32_t nstages;     // radix stages of the M-point FFT
+  int32_t radix[kMaxStages];
+  int64_t L_in, L_out; // elements along the transform axis (in/out types)
+  int64_t n;           // logical transform length (real length for r2c/c2r)
+  int64_t M;           // FFT length per line
+  int64_t sk_in, sk_out;
+  int64_t ldim[kMaxDims];
+  int64_t lsin[kMaxDims];
+  int64_t lsout[kMaxDims];
+  int64_t ntiles;      // ceil(ldim[0]/T) * prod(ldim[1..D-1])
+  int64_t Mp;          // LDS line pitch (complex elements) of the work buffers
+  int64_t raw_pitch;   // LDS line pitch (scalars) of the raw input tile
+  int64_t lds_w1_off;  // byte offset of work buffer 1 (== raw tile) in LDS
+  double scale;        // output multiplier (1 unless ldiv! is fused)
+  int32_t do_scale;
+  int32_t pad_;
+};
+
+template <typename R>
+struct cpx {
+  R re, im;
+};
+
+template <typename R>
+PFFT_HD cpx<R> mk(R a, R b) { cpx<R> z; z.re = a; z.im = b; return z; }
+template <typename R>
+PFFT_HD cpx<R> operator+(cpx<R> a, cpx<R> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename R>
+PFFT_HD cpx<R> operator-(cpx<R> a, cpx<R> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename R>
+PFFT_HD cpx<R> operator*(cpx<R> a, cpx<R> b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+PFFT_HD cpx<R> scl(cpx<R> a, R s) { return mk(a.re * s, a.im * s); }
+template <typename R>
+PFFT_HD cpx<R> conj(cpx<R> a) { return mk(a.re, -a.im); }
+// multiply by -i (forward) or +i (backward)
+template <typename R>
+PFFT_HD cpx<R> mul_mi(cpx<R> a) { return mk(a.im, -a.re); }
+template <typename R>
+PFFT_HD cpx<R> mul_pi(cpx<R> a) { return mk(-a.im, a.re); }
+
+}  // namespace pfft

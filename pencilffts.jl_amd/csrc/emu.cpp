@@ -1,0 +1,98 @@
+// Host emulator of the generic pass, for the CPU test-suite only.
+//
+// Runs the exact phase functions of generic_pass.hpp that the GPU kernel runs
+// (same pass descriptors, same twiddle tables rounded to the plan precision),
+// with the workgroup's threads executed one after another between barriers.
+// It is built into its own library (libpencilffts_emu.so) that only tests
+// load; the product library has no host execution path.
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pfft.h"
+#include "generic_pass.hpp"
+#include "plan.hpp"
+
+using namespace pfft;
+
+struct pfft_emu_plan {
+  Plan p;
+};
+
+template <typename R>
+static void emu_pass(const PassPlan& pp, const PassArgs& a) {
+  std::vector<char> lds((size_t)pp.lds_bytes + 64);
+  const int nthr = pp.threads;
+  for (int64_t tile = 0; tile < a.ntiles; ++tile) {
+    TileCtx c;
+    tile_init(a, tile, c);
+    for (int t = 0; t < nthr; ++t) phase_load<R>(a, c, t, nthr, lds.data());
+    for (int t = 0; t < nthr; ++t) phase_pre<R>(a, c, t, nthr, lds.data());
+    for (int s = 0; s < a.nstages; ++s)
+      for (int t = 0; t < nthr; ++t) phase_stage<R>(a, c, s, t, nthr, lds.data());
+    for (int t = 0; t < nthr; ++t) phase_post<R>(a, c, t, nthr, lds.data());
+  }
+}
+
+extern "C" {
+
+__attribute__((visibility("default"))) int pfft_emu_plan_create(
+    pfft_emu_plan** out, int32_t ndims, const int64_t* dims, const int32_t* kinds,
+    int32_t real_type, int32_t permute, int32_t n_extra, const int64_t* extra, uint32_t flags) {
+  *out = nullptr;
+  pfft_emu_plan* h = new pfft_emu_plan();
+  int r = plan_build(h->p, ndims, dims, kinds, real_type, permute, n_extra, extra,
+                     flags | PFFT_FLAG_GENERIC_ONLY);
+  if (r) {
+    delete h;
+    return r;
+  }
+  *out = h;
+  return PFFT_OK;
+}
+
+__attribute__((visibility("default"))) int pfft_emu_plan_destroy(pfft_emu_plan* h) {
+  delete h;
+  return PFFT_OK;
+}
+
+__attribute__((visibility("default"))) const char* pfft_emu_last_error(void) {
+  return last_error();
+}
+
+__attribute__((visibility("default"))) int pfft_emu_nbytes(const pfft_emu_plan* h,
+                                                          size_t* in_b, size_t* out_b) {
+  *in_b = h->p.in_bytes;
+  *out_b = h->p.out_bytes;
+  return PFFT_OK;
+}
+
+// Host buffers in the same layouts as pfft_execute's device buffers.
+__attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, int32_t dir,
+                                                           const void* in, void* out) {
+  Plan& p = h->p;
+  if (dir != PFFT_FORWARD && dir != PFFT_BACKWARD && dir != PFFT_BACKWARD_SCALED)
+    return fail(PFFT_ERR_ARGUMENT, "unknown direction");
+  std::vector<char> work(p.work_bytes + 64);
+  const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+  void* bufs[3] = {nullptr, work.data(), work.data() + p.work0_bytes};
+  const int np = (int)passes.size();
+  for (int i = 0; i < np; ++i) {
+    const PassPlan& pp = passes[i];
+    PassArgs a = pp.args;
+    a.in = pp.src == 0 ? in : bufs[pp.src];
+    a.out = pp.dst == 0 ? out : bufs[pp.dst];
+    a.tw = pp.tw_id >= 0 ? (const void*)p.tables[pp.tw_id].host.data() : nullptr;
+    a.aux = pp.aux_id >= 0 ? (const void*)p.tables[pp.aux_id].host.data() : nullptr;
+    if (dir == PFFT_BACKWARD_SCALED && i == np - 1) {
+      a.do_scale = 1;
+      a.scale = 1.0 / p.scale;
+    }
+    if (p.real_type == PFFT_F32) emu_pass<float>(pp, a);
+    else emu_pass<double>(pp, a);
+  }
+  return PFFT_OK;
+}
+
+}  // extern "C"

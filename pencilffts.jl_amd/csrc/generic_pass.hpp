@@ -1,0 +1,386 @@
+// Generic pass: any line length, any transform kind, float or double.
+//
+// One workgroup handles one tile of T lines.  The tile goes through LDS in
+// four phases separated by workgroup barriers:
+//   load   -- global -> raw tile in LDS (coalesced along the line when the
+//             transform axis is contiguous, across the T lines otherwise)
+//   pre    -- raw tile -> complex work lines (per-kind reorder / twiddle)
+//   stages -- Stockham mixed-radix FFT, ping-pong between two LDS buffers
+//   post   -- work lines -> output elements (per-kind twiddle / split), with
+//             the store going straight to the next pencil's memory order
+// Every phase is a plain function of (thread id, thread count, LDS base), so
+// the same code runs in the GPU kernel and in the host emulator used by the
+// CPU test-suite.
+#pragma once
+
+#include "common.hpp"
+
+namespace pfft {
+
+struct TileCtx {
+  int64_t in_base;   // element offset of the tile's first line (input)
+  int64_t out_base;  // element offset of the tile's first line (output)
+  int32_t Tact;      // lines present in this tile
+  int32_t Tw;        // work lines (two per line for complex R2R data)
+};
+
+PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
+  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
+  const int64_t b0 = tile % nb0;
+  int64_t rest = tile / nb0;
+  int64_t ib = b0 * a.T * a.lsin[0];
+  int64_t ob = b0 * a.T * a.lsout[0];
+  for (int d = 1; d < a.D; ++d) {
+    const int64_t m = rest % a.ldim[d];
+    rest /= a.ldim[d];
+    ib += m * a.lsin[d];
+    ob += m * a.lsout[d];
+  }
+  c.in_base = ib;
+  c.out_base = ob;
+  const int64_t left = a.ldim[0] - b0 * a.T;
+  c.Tact = (int32_t)(left < a.T ? left : a.T);
+  c.Tw = a.r2r_cplx ? 2 * c.Tact : c.Tact;
+}
+
+// ---- phase 1: load -------------------------------------------------------
+template <typename R>
+PFFT_HD void phase_load(const PassArgs& a, const TileCtx& c, int tid, int nthr,
+                        char* lds) {
+  R* raw = (R*)(lds + a.lds_w1_off);
+  const uint32_t L = (uint32_t)a.L_in;
+  const uint32_t Tact = (uint32_t)c.Tact;
+  const uint32_t count = Tact * L;
+  const bool along = (a.sk_in == 1);
+  for (uint32_t e = tid; e < count; e += nthr) {
+    uint32_t t, j;
+    if (along) { t = e / L; j = e - t * L; }
+    else { j = e / Tact; t = e - j * Tact; }
+    const int64_t g = c.in_base + (int64_t)t * a.lsin[0] + (int64_t)j * a.sk_in;
+    if (a.in_cplx) {
+      const cpx<R> v = ((const cpx<R>*)a.in)[g];
+      *(cpx<R>*)(raw + t * a.raw_pitch + 2 * j) = v;
+    } else {
+      raw[t * a.raw_pitch + j] = ((const R*)a.in)[g];
+    }
+  }
+}
+
+// real scalar j of work line w (real data, or one part of complex R2R data)
+template <typename R>
+PFFT_HD R raw_real(const PassArgs& a, const R* raw, uint32_t w, int64_t j) {
+  if (a.r2r_cplx) return raw[(w >> 1) * a.raw_pitch + 2 * j + (w & 1)];
+  return raw[w * a.raw_pitch + j];
+}
+template <typename R>
+PFFT_HD cpx<R> raw_cplx(const PassArgs& a, const R* raw, uint32_t t, int64_t j) {
+  return *(const cpx<R>*)(raw + t * a.raw_pitch + 2 * j);
+}
+
+// ---- phase 2: pre-process into work buffer 0 ------------------------------
+template <typename R>
+PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
+                       char* lds) {
+  if (a.kind == L_COPY) return;
+  const R* raw = (const R*)(lds + a.lds_w1_off);
+  cpx<R>* W = (cpx<R>*)lds;
+  const cpx<R>* aux = (const cpx<R>*)a.aux;
+  const uint32_t M = (uint32_t)a.M;
+  const int64_t n = a.n;
+  const uint32_t count = (uint32_t)c.Tw * M;
+  const R zero = R(0);
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / M;
+    const uint32_t j = e - w * M;
+    cpx<R> v;
+    switch (a.kind) {
+      case L_C2C:
+        v = raw_cplx(a, raw, w, j);
+        break;
+      case L_R2C_HALF:
+        v = mk(raw_real(a, raw, w, 2 * j), raw_real(a, raw, w, 2 * j + 1));
+        break;
+      case L_R2C_FULL:
+      case L_DHT:
+        v = mk(raw_real(a, raw, w, j), zero);
+        break;
+      case L_C2R_HALF: {
+        cpx<R> A = raw_cplx(a, raw, w, j);
+        cpx<R> B = conj(raw_cplx(a, raw, w, M - j));
+        if (j == 0) { A.im = zero; B.im = zero; }  // Im of DC and Nyquist ignored
+        const cpx<R> E = A + B;
+        const cpx<R> O = (A - B) * conj(aux[j]);
+        v = E + mul_pi(O);
+        break;
+      }
+      case L_C2R_FULL: {
+        if (j == 0) v = mk(raw_cplx(a, raw, w, 0).re, zero);
+        else if (2 * (int64_t)j < n) v = raw_cplx(a, raw, w, j);
+        else v = conj(raw_cplx(a, raw, w, n - j));
+        break;
+      }
+      case L_DCT1:
+        v = mk(raw_real(a, raw, w, j < n ? j : M - j), zero);
+        break;
+      case L_DCT2:
+      case L_DST2: {
+        const int64_t h = (n + 1) / 2;
+        if ((int64_t)j < h) v = mk(raw_real(a, raw, w, 2 * j), zero);
+        else {
+          const R x = raw_real(a, raw, w, 2 * n - 2 * (int64_t)j - 1);
+          v = mk(a.kind == L_DST2 ? -x : x, zero);
+        }
+        break;
+      }
+      case L_DCT3: {
+        const R xk = raw_real(a, raw, w, j);
+        const R xnk = j == 0 ? zero : raw_real(a, raw, w, n - j);
+        v = conj(aux[j]) * mk(xk, -xnk);
+        break;
+      }
+      case L_DST3: {
+        const R xk = raw_real(a, raw, w, n - 1 - j);
+        const R xnk = j == 0 ? zero : raw_real(a, raw, w, (int64_t)j - 1);
+        v = conj(aux[j]) * mk(xk, -xnk);
+        break;
+      }
+      case L_DCT4:
+        v = (int64_t)j < n ? scl(aux[j], raw_real(a, raw, w, j)) : mk(zero, zero);
+        break;
+      case L_DST4:
+        v = (int64_t)j < n ? scl(aux[j], raw_real(a, raw, w, n - 1 - j))
+                           : mk(zero, zero);
+        break;
+      case L_DST1: {
+        const int64_t jj = j;
+        if (jj == 0 || jj == n + 1) v = mk(zero, zero);
+        else if (jj <= n) v = mk(raw_real(a, raw, w, jj - 1), zero);
+        else v = mk(-raw_real(a, raw, w, (int64_t)M - jj - 1), zero);
+        break;
+      }
+      default:
+        v = mk(zero, zero);
+    }
+    W[w * a.Mp + j] = v;
+  }
+}
+
+// ---- phase 3: one Stockham stage ------------------------------------------
+// Radix-R stage with Ns = product of the previous radices (autosort form):
+//   v_r = src[j + r M/R] * w_M^(r (j mod Ns) M/(Ns R)),  y = DFT_R(v),
+//   dst[(j / Ns) Ns R + (j mod Ns) + r Ns] = y_r.
+// Backward stages use DFT_bwd(v) = conj(DFT_fwd(conj(v))).
+template <typename R>
+PFFT_HD void dft2(cpx<R>* v) {
+  const cpx<R> a = v[0], b = v[1];
+  v[0] = a + b; v[1] = a - b;
+}
+template <typename R>
+PFFT_HD void dft3(cpx<R>* v) {
+  const R c = R(-0.5), s = R(0.86602540378443864676372317075294);
+  const cpx<R> t = v[1] + v[2];
+  const cpx<R> d = v[1] - v[2];
+  const cpx<R> m = v[0] + scl(t, c);
+  const cpx<R> u = mk(d.im * s, -d.re * s);  // -i s d
+  v[0] = v[0] + t;
+  v[1] = m + u;
+  v[2] = m - u;
+}
+template <typename R>
+PFFT_HD void dft4(cpx<R>* v) {
+  const cpx<R> t0 = v[0] + v[2], t1 = v[0] - v[2];
+  const cpx<R> t2 = v[1] + v[3], t3 = mul_mi(v[1] - v[3]);
+  v[0] = t0 + t2; v[2] = t0 - t2; v[1] = t1 + t3; v[3] = t1 - t3;
+}
+template <typename R>
+PFFT_HD void dft5(cpx<R>* v) {
+  const R c1 = R(0.30901699437494742410229341718282);
+  const R c2 = R(-0.80901699437494742410229341718282);
+  const R s1 = R(0.95105651629515357211643933337938);
+  const R s2 = R(0.58778525229247312916870595463907);
+  const cpx<R> a0 = v[0];
+  const cpx<R> sa = v[1] + v[4], da = v[1] - v[4];
+  const cpx<R> sb = v[2] + v[3], db = v[2] - v[3];
+  const cpx<R> p1 = a0 + scl(sa, c1) + scl(sb, c2);
+  const cpx<R> p2 = a0 + scl(sa, c2) + scl(sb, c1);
+  const cpx<R> q1 = scl(da, s1) + scl(db, s2);   // times -i
+  const cpx<R> q2 = scl(da, s2) - scl(db, s1);   // times -i
+  v[0] = a0 + sa + sb;
+  v[1] = p1 + mul_mi(q1);
+  v[4] = p1 - mul_mi(q1);
+  v[2] = p2 + mul_mi(q2);
+  v[3] = p2 - mul_mi(q2);
+}
+template <typename R>
+PFFT_HD void dft8(cpx<R>* v) {
+  const R h = R(0.70710678118654752440084436210485);
+  cpx<R> b[4], c[4];
+  for (int r = 0; r < 4; ++r) { b[r] = v[r] + v[r + 4]; c[r] = v[r] - v[r + 4]; }
+  c[1] = mk((c[1].re + c[1].im) * h, (c[1].im - c[1].re) * h);   // * (1-i)/sqrt2
+  c[2] = mul_mi(c[2]);                                           // * -i
+  c[3] = mk((c[3].im - c[3].re) * h, -(c[3].re + c[3].im) * h);  // * (-1-i)/sqrt2
+  dft4(b); dft4(c);
+  v[0] = b[0]; v[2] = b[1]; v[4] = b[2]; v[6] = b[3];
+  v[1] = c[0]; v[3] = c[1]; v[5] = c[2]; v[7] = c[3];
+}
+
+template <typename R, int RAD>
+PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns,
+                         const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  const uint32_t M = (uint32_t)a.M;
+  const uint32_t nb = M / RAD;                 // butterflies per line
+  const uint32_t step = (uint32_t)(M / (Ns * RAD));
+  const uint32_t count = (uint32_t)c.Tw * nb;
+  const bool bwd = a.bwd != 0;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / nb;
+    const uint32_t j = e - w * nb;
+    const uint32_t jj = j % (uint32_t)Ns;
+    const cpx<R>* s = src + w * a.Mp;
+    cpx<R> v[RAD];
+    for (int r = 0; r < RAD; ++r) {
+      cpx<R> x = s[j + r * nb];
+      if (bwd) x = conj(x);
+      v[r] = r == 0 ? x : x * tw[r * jj * step];
+    }
+    if (RAD == 2) dft2(v);
+    else if (RAD == 3) dft3(v);
+    else if (RAD == 4) dft4(v);
+    else if (RAD == 5) dft5(v);
+    else if (RAD == 8) dft8(v);
+    cpx<R>* d = dst + w * a.Mp + (j / (uint32_t)Ns) * (uint32_t)Ns * RAD + jj;
+    for (int r = 0; r < RAD; ++r) d[r * Ns] = bwd ? conj(v[r]) : v[r];
+  }
+}
+
+// Any radix (e.g. a prime factor 7, 11, 41): one work item per output.
+template <typename R>
+PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
+                       const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  const uint32_t M = (uint32_t)a.M;
+  const uint32_t nb = M / RAD;
+  const uint32_t step = (uint32_t)(M / (Ns * RAD));
+  const uint32_t count = (uint32_t)c.Tw * M;
+  const bool bwd = a.bwd != 0;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / M;
+    const uint32_t rem = e - w * M;
+    const uint32_t q = rem / nb;   // output index within the butterfly
+    const uint32_t j = rem - q * nb;
+    const uint32_t jj = j % (uint32_t)Ns;
+    const cpx<R>* s = src + w * a.Mp;
+    cpx<R> acc = mk(R(0), R(0));
+    for (int r = 0; r < RAD; ++r) {
+      cpx<R> x = s[j + r * nb];
+      if (bwd) x = conj(x);
+      // twiddle of the stage times the DFT_R kernel entry w_R^(r q)
+      const uint32_t idx = (r * jj * step + ((uint32_t)(r * q) % RAD) * nb) % M;
+      acc = acc + x * tw[idx];
+    }
+    dst[w * a.Mp + (j / (uint32_t)Ns) * (uint32_t)Ns * RAD + jj + q * Ns] =
+        bwd ? conj(acc) : acc;
+  }
+}
+
+template <typename R>
+PFFT_HD void phase_stage(const PassArgs& a, const TileCtx& c, int s, int tid,
+                         int nthr, char* lds) {
+  int64_t Ns = 1;
+  for (int i = 0; i < s; ++i) Ns *= a.radix[i];
+  const cpx<R>* src = (const cpx<R>*)(lds + ((s & 1) ? a.lds_w1_off : 0));
+  cpx<R>* dst = (cpx<R>*)(lds + ((s & 1) ? 0 : a.lds_w1_off));
+  switch (a.radix[s]) {
+    case 2: stage_fixed<R, 2>(a, c, Ns, src, dst, tid, nthr); break;
+    case 3: stage_fixed<R, 3>(a, c, Ns, src, dst, tid, nthr); break;
+    case 4: stage_fixed<R, 4>(a, c, Ns, src, dst, tid, nthr); break;
+    case 5: stage_fixed<R, 5>(a, c, Ns, src, dst, tid, nthr); break;
+    case 8: stage_fixed<R, 8>(a, c, Ns, src, dst, tid, nthr); break;
+    default: stage_any<R>(a, c, Ns, a.radix[s], src, dst, tid, nthr); break;
+  }
+}
+
+// ---- phase 4: post-process and store --------------------------------------
+template <typename R>
+PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
+  const cpx<R>* aux = (const cpx<R>*)a.aux;
+  const cpx<R>* L = W + w * a.Mp;
+  const int64_t n = a.n;
+  switch (a.kind) {
+    case L_C2R_HALF: return (k & 1) ? L[k >> 1].im : L[k >> 1].re;
+    case L_C2R_FULL:
+    case L_DCT1: return L[k].re;
+    case L_DCT2: {
+      const cpx<R> z = aux[k] * L[k];
+      return R(2) * z.re;
+    }
+    case L_DST2: {
+      const int64_t kk = n - 1 - k;
+      const cpx<R> z = aux[kk] * L[kk];
+      return R(2) * z.re;
+    }
+    case L_DCT3:
+    case L_DST3: {
+      const int64_t m = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
+      const R y = L[m].re;
+      return (a.kind == L_DST3 && (k & 1)) ? -y : y;
+    }
+    case L_DCT4:
+    case L_DST4: {
+      const cpx<R> z = aux[n + k] * L[k];
+      const R y = R(2) * z.re;
+      return (a.kind == L_DST4 && (k & 1)) ? -y : y;
+    }
+    case L_DST1: return -L[k + 1].im;
+    case L_DHT: return L[k].re - L[k].im;
+    default: return R(0);
+  }
+}
+
+template <typename R>
+PFFT_HD cpx<R> post_cplx(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
+  const cpx<R>* L = W + w * a.Mp;
+  if (a.kind == L_R2C_HALF) {
+    const cpx<R>* aux = (const cpx<R>*)a.aux;
+    const int64_t M = a.M;
+    const cpx<R> Z = L[k == M ? 0 : k];
+    const cpx<R> Zc = conj(L[k == 0 ? 0 : M - k]);
+    const cpx<R> E = scl(Z + Zc, R(0.5));
+    const cpx<R> O = scl(mul_mi(Z - Zc), R(0.5));
+    return E + aux[k] * O;
+  }
+  return L[k];  // C2C, R2C_FULL
+}
+
+template <typename R>
+PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
+                        char* lds) {
+  const cpx<R>* W = (const cpx<R>*)(lds + ((a.nstages & 1) ? a.lds_w1_off : 0));
+  const R* raw = (const R*)(lds + a.lds_w1_off);
+  const uint32_t L = (uint32_t)a.L_out;
+  const uint32_t Tact = (uint32_t)c.Tact;
+  const uint32_t count = Tact * L;
+  const bool along = (a.sk_out == 1);
+  const R sc = (R)a.scale;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    uint32_t t, k;
+    if (along) { t = e / L; k = e - t * L; }
+    else { k = e / Tact; t = e - k * Tact; }
+    const int64_t g = c.out_base + (int64_t)t * a.lsout[0] + (int64_t)k * a.sk_out;
+    if (a.out_cplx) {
+      cpx<R> y;
+      if (a.kind == L_COPY) y = raw_cplx(a, raw, t, k);
+      else if (a.r2r_cplx) y = mk(post_real(a, W, 2 * t, k), post_real(a, W, 2 * t + 1, k));
+      else y = post_cplx(a, W, t, k);
+      if (a.do_scale) y = scl(y, sc);
+      ((cpx<R>*)a.out)[g] = y;
+    } else {
+      R y = a.kind == L_COPY ? raw[t * a.raw_pitch + k] : post_real(a, W, t, k);
+      if (a.do_scale) y = y * sc;
+      ((R*)a.out)[g] = y;
+    }
+  }
+}
+
+}  // namespace pfft

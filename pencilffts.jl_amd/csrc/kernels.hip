@@ -1,0 +1,202 @@
+// Device side of the transform pipeline: kernels, device tables and the
+// execution layer (the reference's _apply_plans_out_of_place!,
+// src/operations.jl:227-257, with the transposes fused into the passes and
+// ldiv!'s normalisation, operations.jl:50,64-66, fused into the last pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/pfft.h"
+#include "fast_pass.hpp"
+#include "generic_pass.hpp"
+#include "plan.hpp"
+
+namespace pfft {
+
+// ---- generic LDS pass ---------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kBlockThreads) void k_generic(PassArgs a) {
+  extern __shared__ __align__(16) char lds[];
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    TileCtx c;
+    tile_init(a, tile, c);
+    phase_load<R>(a, c, threadIdx.x, blockDim.x, lds);
+    __syncthreads();
+    phase_pre<R>(a, c, threadIdx.x, blockDim.x, lds);
+    __syncthreads();
+    for (int s = 0; s < a.nstages; ++s) {
+      phase_stage<R>(a, c, s, threadIdx.x, blockDim.x, lds);
+      __syncthreads();
+    }
+    phase_post<R>(a, c, threadIdx.x, blockDim.x, lds);
+    __syncthreads();
+  }
+}
+
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(PFFT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DeviceState {
+  int device = -1;
+  void* work = nullptr;
+  size_t work_size = 0;
+  bool timing = false;
+  hipEvent_t ev[2 * kMaxDims + 2] = {};
+  int n_ev = 0;
+};
+
+static int set_lds_limit(const void* fn, int64_t bytes) {
+  if (bytes > 64 * 1024)
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return PFFT_OK;
+}
+
+int device_prepare(Plan& p) {
+  if (p.dev) return PFFT_OK;
+  DeviceState* d = new DeviceState();
+  if (hipGetDevice(&d->device) != hipSuccess) {
+    delete d;
+    return fail(PFFT_ERR_DEVICE, "no HIP device available");
+  }
+  for (Table& t : p.tables) {
+    const size_t bytes = t.host.size();
+    hipError_t e = hipMalloc(&t.dev, bytes);
+    if (e != hipSuccess) {
+      delete d;
+      return fail(PFFT_ERR_ALLOC, std::string("hipMalloc(tables): ") + hipGetErrorString(e));
+    }
+    e = hipMemcpy(t.dev, t.host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      delete d;
+      return fail(PFFT_ERR_DEVICE, std::string("hipMemcpy(tables): ") + hipGetErrorString(e));
+    }
+  }
+  for (auto* list : {&p.fwd, &p.bwd}) {
+    for (PassPlan& pp : *list) {
+      int r = PFFT_OK;
+      if (pp.variant == 0) {
+        r = p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
+                                    : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);
+      } else {
+        r = fast_prepare(pp);
+      }
+      if (r) {
+        delete d;
+        return r;
+      }
+    }
+  }
+  p.dev = d;
+  return PFFT_OK;
+}
+
+void device_release(Plan& p) {
+  for (Table& t : p.tables) {
+    if (t.dev) (void)hipFree(t.dev);
+    t.dev = nullptr;
+  }
+  if (p.dev) {
+    if (p.dev->work) (void)hipFree(p.dev->work);
+    for (int i = 0; i < p.dev->n_ev; ++i) (void)hipEventDestroy(p.dev->ev[i]);
+    delete p.dev;
+    p.dev = nullptr;
+  }
+}
+
+int device_set_timing(Plan& p, bool on) {
+  int r = device_prepare(p);
+  if (r) return r;
+  DeviceState* d = p.dev;
+  if (on && d->n_ev == 0) {
+    d->n_ev = (int)std::max(p.fwd.size(), p.bwd.size()) + 1;
+    for (int i = 0; i < d->n_ev; ++i) HIP_TRY(hipEventCreate(&d->ev[i]));
+  }
+  d->timing = on;
+  return PFFT_OK;
+}
+
+int device_stage_times(Plan& p, int n, float* ms) {
+  if (!p.dev || !p.dev->timing) return fail(PFFT_ERR_ARGUMENT, "timing is not enabled on this plan");
+  DeviceState* d = p.dev;
+  const int np = d->n_ev - 1;
+  if (n < np) return fail(PFFT_ERR_ARGUMENT, "buffer too small for the stage times");
+  HIP_TRY(hipEventSynchronize(d->ev[np]));
+  for (int i = 0; i < np; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]));
+  return PFFT_OK;
+}
+
+static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  const char* x = (const char*)a;
+  const char* y = (const char*)b;
+  return x < y + nb && y < x + na;
+}
+
+int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
+                   hipStream_t stream) {
+  if (dir != PFFT_FORWARD && dir != PFFT_BACKWARD && dir != PFFT_BACKWARD_SCALED)
+    return fail(PFFT_ERR_ARGUMENT, "unknown direction " + std::to_string(dir));
+  if (!in || !out) return fail(PFFT_ERR_ARGUMENT, "null array pointer");
+  const size_t nin = dir == PFFT_FORWARD ? p.in_bytes : p.out_bytes;
+  const size_t nout = dir == PFFT_FORWARD ? p.out_bytes : p.in_bytes;
+  if (overlaps(in, nin, out, nout))
+    return fail(PFFT_ERR_ARGUMENT, "out-of-place plan applied to aliased data");
+  int r = device_prepare(p);
+  if (r) return r;
+  DeviceState* d = p.dev;
+  char* wk = (char*)work;
+  if (p.work_bytes) {
+    if (!wk) {
+      if (d->work_size < p.work_bytes) {
+        if (d->work) HIP_TRY(hipFree(d->work));
+        d->work = nullptr;
+        hipError_t e = hipMalloc(&d->work, p.work_bytes);
+        if (e != hipSuccess)
+          return fail(PFFT_ERR_ALLOC, std::string("hipMalloc(work): ") + hipGetErrorString(e));
+        d->work_size = p.work_bytes;
+      }
+      wk = (char*)d->work;
+    } else if (work_bytes < p.work_bytes) {
+      return fail(PFFT_ERR_ARGUMENT, "workspace too small: need " + std::to_string(p.work_bytes) +
+                                         " bytes, got " + std::to_string(work_bytes));
+    } else if (overlaps(wk, p.work_bytes, in, nin) || overlaps(wk, p.work_bytes, out, nout)) {
+      return fail(PFFT_ERR_ARGUMENT, "workspace overlaps the input or output array");
+    }
+  }
+  const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+  void* bufs[3] = {nullptr, wk, wk ? wk + p.work0_bytes : nullptr};
+  const int np = (int)passes.size();
+  if (d->timing) HIP_TRY(hipEventRecord(d->ev[0], stream));
+  for (int i = 0; i < np; ++i) {
+    PassArgs a = passes[i].args;
+    const PassPlan& pp = passes[i];
+    a.in = pp.src == 0 ? in : bufs[pp.src];
+    a.out = pp.dst == 0 ? out : bufs[pp.dst];
+    a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
+    a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+    if (dir == PFFT_BACKWARD_SCALED && i == np - 1) {
+      a.do_scale = 1;
+      a.scale = 1.0 / p.scale;
+    }
+    if (pp.variant == 0) {
+      if (p.real_type == PFFT_F32)
+        hipLaunchKernelGGL(k_generic<float>, dim3((unsigned)pp.grid), dim3(pp.threads),
+                           (size_t)pp.lds_bytes, stream, a);
+      else
+        hipLaunchKernelGGL(k_generic<double>, dim3((unsigned)pp.grid), dim3(pp.threads),
+                           (size_t)pp.lds_bytes, stream, a);
+    } else {
+      r = fast_launch(pp, a, stream);
+      if (r) return r;
+    }
+    HIP_TRY(hipGetLastError());
+    if (d->timing) HIP_TRY(hipEventRecord(d->ev[i + 1], stream));
+  }
+  return PFFT_OK;
+}
+
+}  // namespace pfft

@@ -1,0 +1,652 @@
+// PLAN layer of the transform pipeline, restated in C++.
+//
+// Follows PencilFFTs.jl v0.15.3:
+//   transform metadata      src/Transforms/{Transforms,c2c,r2c,r2r,no_transform}.jl
+//   GlobalFFTParams         src/global_params.jl:50-62, input type :81-95
+//   pencil chain            src/plans.jl:397-521
+//   1D plans, binv, scale   src/plans.jl:523-553, product :268
+// and then lowers every stage to one fused pass (transform + the transpose
+// into the next pencil's memory order).
+#include "plan.hpp"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <sstream>
+
+#include "../../include/pfft.h"
+
+namespace pfft {
+
+// ---- errors ----------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_last_error = msg;
+}
+int fail(int code, const std::string& msg) {
+  set_error(code, msg);
+  return code;
+}
+const char* last_error() { return g_last_error.c_str(); }
+
+// ---- transform metadata ------------------------------------------------------
+static const char* kR2RNames[11] = {"R2HC",    "HC2R",    "DHT",     "REDFT00",
+                                    "REDFT01", "REDFT10", "REDFT11", "RODFT00",
+                                    "RODFT01", "RODFT10", "RODFT11"};
+
+int32_t kind_base(int32_t kind) { return kind & ~PFFT_INPLACE; }
+bool kind_inplace(int32_t kind) { return (kind & PFFT_INPLACE) != 0; }
+
+static bool is_r2r(int32_t b) { return b >= PFFT_R2R && b <= PFFT_R2R + PFFT_RODFT11; }
+static bool is_c2c(int32_t b) { return b == PFFT_FFT || b == PFFT_BFFT; }
+static bool is_c2r(int32_t b) { return b == PFFT_BRFFT_EVEN || b == PFFT_BRFFT_ODD; }
+
+bool kind_valid(int32_t kind, std::string* why) {
+  if (kind & ~(PFFT_INPLACE | 0xff)) {
+    if (why) *why = "unknown transform kind " + std::to_string(kind);
+    return false;
+  }
+  const int32_t b = kind_base(kind);
+  if (b == PFFT_NOTRANSFORM || is_c2c(b) || b == PFFT_RFFT || is_c2r(b)) return true;
+  if (is_r2r(b)) {
+    const int32_t k = b - PFFT_R2R;
+    if (k == PFFT_R2HC || k == PFFT_HC2R) {
+      // r2r.jl:40-46
+      if (why) *why = std::string("unsupported r2r transform kind: ") + kR2RNames[k];
+      return false;
+    }
+    return true;
+  }
+  if (why) *why = "unknown transform kind " + std::to_string(kind);
+  return false;
+}
+
+std::string kind_name(int32_t kind) {
+  const int32_t b = kind_base(kind);
+  const char* bang = kind_inplace(kind) ? "!" : "";
+  switch (b) {
+    case PFFT_NOTRANSFORM: return std::string("NoTransform") + bang;
+    case PFFT_FFT: return std::string("FFT") + bang;
+    case PFFT_BFFT: return std::string("BFFT") + bang;
+    case PFFT_RFFT: return std::string("RFFT") + bang;
+    case PFFT_BRFFT_EVEN: return std::string("BRFFT") + bang + "{even}";
+    case PFFT_BRFFT_ODD: return std::string("BRFFT") + bang + "{odd}";
+    default: break;
+  }
+  if (is_r2r(b)) return std::string("R2R") + bang + "{" + kR2RNames[b - PFFT_R2R] + "}";
+  return "?";
+}
+
+std::string kind_type_name(int32_t kind) {
+  const int32_t b = kind_base(kind);
+  const char* bang = kind_inplace(kind) ? "!" : "";
+  std::string base;
+  switch (b) {
+    case PFFT_NOTRANSFORM: base = "NoTransform"; break;
+    case PFFT_FFT: base = "FFT"; break;
+    case PFFT_BFFT: base = "BFFT"; break;
+    case PFFT_RFFT: base = "RFFT"; break;
+    case PFFT_BRFFT_EVEN:
+    case PFFT_BRFFT_ODD: base = "BRFFT"; break;
+    default:
+      if (is_r2r(b))
+        return std::string("PencilFFTs.Transforms.R2R") + bang + "{" +
+               std::to_string(b - PFFT_R2R) + "}";
+      base = "?";
+  }
+  return "PencilFFTs.Transforms." + base + bang;
+}
+
+std::string eltype_name(int32_t e) {
+  switch (e) {
+    case PFFT_F32: return "Float32";
+    case PFFT_F64: return "Float64";
+    case PFFT_C32: return "ComplexF32";
+    case PFFT_C64: return "ComplexF64";
+    default: return "Nothing";
+  }
+}
+
+size_t eltype_bytes(int32_t e) {
+  switch (e) {
+    case PFFT_F32: return 4;
+    case PFFT_F64: return 8;
+    case PFFT_C32: return 8;
+    case PFFT_C64: return 16;
+    default: return 0;
+  }
+}
+
+static bool eltype_is_complex(int32_t e) { return e == PFFT_C32 || e == PFFT_C64; }
+static int32_t eltype_real(int32_t e) { return (e == PFFT_C32 || e == PFFT_F32) ? PFFT_F32 : PFFT_F64; }
+static int32_t eltype_complex(int32_t e) { return eltype_real(e) == PFFT_F32 ? PFFT_C32 : PFFT_C64; }
+
+// c2c.jl:49, r2c.jl:74-75, r2r.jl:82, no_transform.jl:23
+int length_output(int32_t kind, int64_t n, int64_t* out) {
+  std::string why;
+  if (!kind_valid(kind, &why)) return fail(PFFT_ERR_ARGUMENT, why);
+  const int32_t b = kind_base(kind);
+  if (b == PFFT_RFFT) *out = n / 2 + 1;
+  else if (b == PFFT_BRFFT_EVEN) *out = 2 * n - 2;
+  else if (b == PFFT_BRFFT_ODD) *out = 2 * n - 1;
+  else *out = n;
+  return PFFT_OK;
+}
+
+// c2c.jl:58-61, r2c.jl:99-102 (BRFFT(d) = BRFFT(iseven(d)), r2c.jl:66),
+// r2r.jl:103-131, no_transform.jl:21
+int binv(int32_t kind, int64_t d, int32_t* inv) {
+  std::string why;
+  if (!kind_valid(kind, &why)) return fail(PFFT_ERR_ARGUMENT, why);
+  const int32_t ip = kind & PFFT_INPLACE;
+  const int32_t b = kind_base(kind);
+  int32_t r;
+  switch (b) {
+    case PFFT_FFT: r = PFFT_BFFT; break;
+    case PFFT_BFFT: r = PFFT_FFT; break;
+    case PFFT_RFFT: r = (d % 2 == 0) ? PFFT_BRFFT_EVEN : PFFT_BRFFT_ODD; break;
+    case PFFT_BRFFT_EVEN:
+    case PFFT_BRFFT_ODD: r = PFFT_RFFT; break;
+    case PFFT_NOTRANSFORM: r = PFFT_NOTRANSFORM; break;
+    default: {
+      const int32_t k = b - PFFT_R2R;
+      int32_t ki = k;
+      if (k == PFFT_REDFT01) ki = PFFT_REDFT10;
+      else if (k == PFFT_REDFT10) ki = PFFT_REDFT01;
+      else if (k == PFFT_RODFT01) ki = PFFT_RODFT10;
+      else if (k == PFFT_RODFT10) ki = PFFT_RODFT01;
+      r = PFFT_R2R + ki;
+    }
+  }
+  *inv = r | ip;
+  return PFFT_OK;
+}
+
+// Scale factor of a transform over one dimension whose input length is n:
+// c2c.jl:71, r2c.jl:104-111 (c2r: the real output length), r2r.jl:96-101,
+// no_transform.jl:25.
+int scale_factor_1d(int32_t kind, int64_t n, int64_t* s) {
+  std::string why;
+  if (!kind_valid(kind, &why)) return fail(PFFT_ERR_ARGUMENT, why);
+  const int32_t b = kind_base(kind);
+  if (b == PFFT_NOTRANSFORM) *s = 1;
+  else if (is_c2c(b) || b == PFFT_RFFT) *s = n;
+  else if (is_c2r(b)) return length_output(kind, n, s);
+  else {
+    const int32_t k = b - PFFT_R2R;
+    if (k == PFFT_REDFT00) *s = 2 * (n - 1);
+    else if (k == PFFT_RODFT00) *s = 2 * (n + 1);
+    else if (k == PFFT_DHT) *s = n;
+    else *s = 2 * n;
+  }
+  return PFFT_OK;
+}
+
+// c2c.jl:51, r2c.jl:80-81, r2r.jl:83, no_transform.jl:24
+int eltype_input(int32_t kind, int32_t real_type, int32_t* out) {
+  std::string why;
+  if (!kind_valid(kind, &why)) return fail(PFFT_ERR_ARGUMENT, why);
+  if (real_type != PFFT_F32 && real_type != PFFT_F64)
+    return fail(PFFT_ERR_ARGUMENT, "real type must be Float32 or Float64");
+  const int32_t b = kind_base(kind);
+  if (is_c2c(b) || is_c2r(b)) *out = eltype_complex(real_type);
+  else if (b == PFFT_RFFT) *out = real_type;
+  else *out = PFFT_NOTHING;
+  return PFFT_OK;
+}
+
+// c2c.jl:50, r2c.jl:77-78, r2r.jl:84, no_transform.jl:22, error Transforms.jl:263-264
+int eltype_output(int32_t kind, int32_t t, int32_t* out) {
+  std::string why;
+  if (!kind_valid(kind, &why)) return fail(PFFT_ERR_ARGUMENT, why);
+  if (t < PFFT_F32 || t > PFFT_C64)
+    return fail(PFFT_ERR_ARGUMENT, "invalid element type code " + std::to_string(t));
+  const int32_t b = kind_base(kind);
+  bool ok = true;
+  int32_t r = t;
+  if (is_c2c(b)) ok = eltype_is_complex(t);
+  else if (b == PFFT_RFFT) { ok = !eltype_is_complex(t); r = eltype_complex(t); }
+  else if (is_c2r(b)) { ok = eltype_is_complex(t); r = eltype_real(t); }
+  if (!ok)
+    return fail(PFFT_ERR_ARGUMENT, "invalid input data type for " + kind_type_name(kind) +
+                                       ": " + eltype_name(t));
+  *out = r;
+  return PFFT_OK;
+}
+
+// ---- tables ------------------------------------------------------------------
+// exp(-2 pi i num / den) with the argument reduced exactly in integers and the
+// trigonometric functions evaluated in long double before rounding.
+static void expm(int64_t num, int64_t den, double* re, double* im) {
+  int64_t r = num % den;
+  if (r < 0) r += den;
+  // fold into [0, den/8] with symmetries so the long-double argument is small
+  const long double tau = 6.283185307179586476925286766559005768L;
+  const long double th = tau * (long double)r / (long double)den;
+  *re = (double)cosl(th);
+  *im = (double)(-sinl(th));
+  // exact values at multiples of quarter turns
+  if ((4 * r) % den == 0) {
+    const int64_t q = (4 * r) / den;
+    const double c[4] = {1, 0, -1, 0}, s[4] = {0, -1, 0, 1};
+    *re = c[q & 3];
+    *im = s[q & 3];
+  } else if ((8 * r) % den == 0) {
+    const double h = 0.70710678118654752440084436210485;
+    const int64_t q = (8 * r) / den;  // odd
+    const double c[8] = {1, h, 0, -h, -1, -h, 0, h};
+    const double s[8] = {0, -h, -1, -h, 0, h, 1, h};
+    *re = c[q & 7];
+    *im = s[q & 7];
+  }
+}
+
+static int add_table(Plan& p, const std::string& key_unused, std::vector<double>&& vals) {
+  (void)key_unused;
+  for (size_t i = 0; i < p.tables.size(); ++i)
+    if (p.tables[i].values == vals) return (int)i;
+  Table t;
+  t.real_bytes = p.real_type == PFFT_F32 ? 4 : 8;
+  t.values = std::move(vals);
+  const size_t n = t.values.size();
+  t.host.resize(n * t.real_bytes);
+  if (t.real_bytes == 4) {
+    float* f = (float*)t.host.data();
+    for (size_t i = 0; i < n; ++i) f[i] = (float)t.values[i];
+  } else {
+    memcpy(t.host.data(), t.values.data(), n * 8);
+  }
+  p.tables.push_back(std::move(t));
+  return (int)p.tables.size() - 1;
+}
+
+static int twiddle_table(Plan& p, int64_t M) {
+  std::vector<double> v(2 * M);
+  for (int64_t k = 0; k < M; ++k) expm(k, M, &v[2 * k], &v[2 * k + 1]);
+  return add_table(p, "tw", std::move(v));
+}
+
+// ---- radix schedule ----------------------------------------------------------
+static int radix_schedule(int64_t M, int32_t* rad) {
+  int ns = 0;
+  int64_t m = M;
+  for (int r : {8, 4, 2}) {
+    while (m > 1 && m % r == 0) { rad[ns++] = r; m /= r; }
+  }
+  for (int r : {3, 5}) {
+    while (m % r == 0) { rad[ns++] = r; m /= r; }
+  }
+  for (int64_t q = 7; m > 1; q += 2) {
+    while (m % q == 0) {
+      if (ns >= kMaxStages) return -1;
+      rad[ns++] = (int32_t)q;
+      m /= q;
+    }
+    if (q * q > m && m > 1) {
+      if (ns >= kMaxStages) return -1;
+      rad[ns++] = (int32_t)m;
+      m = 1;
+    }
+  }
+  return ns;
+}
+
+// ---- layouts -----------------------------------------------------------------
+// Strides (in elements) of each logical dim and of the extra dim for a memory
+// permutation `perm` of the N logical dims with sizes `dims`.
+static void layout_strides(int N, const int32_t* perm, const int64_t* dims, int64_t* stride,
+                           int64_t* stride_extra) {
+  int64_t s = 1;
+  for (int i = 0; i < N; ++i) {
+    stride[perm[i]] = s;
+    s *= dims[perm[i]];
+  }
+  *stride_extra = s;
+}
+
+static void chain_perm(int N, int n, bool permute, int32_t* perm) {
+  // plans.jl:503-521: stage n (0-based) reads data permuted as
+  // (n, 0..n-1, n+1..N-1), the last stage fully reversed; stage 0 unpermuted.
+  for (int i = 0; i < N; ++i) perm[i] = i;
+  if (!permute || n == 0) return;
+  if (n == N - 1) {
+    for (int i = 0; i < N; ++i) perm[i] = N - 1 - i;
+    return;
+  }
+  perm[0] = n;
+  for (int i = 1; i <= n; ++i) perm[i] = i - 1;
+  for (int i = n + 1; i < N; ++i) perm[i] = i;
+}
+
+// Build the pass for transforming logical dim `dim` from (perm_in, dims_in)
+// into (perm_out, dims_out).
+static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t elt_out,
+                      const int32_t* perm_in, const int64_t* dims_in, const int32_t* perm_out,
+                      const int64_t* dims_out, PassPlan& pp) {
+  const int N = p.N;
+  PassArgs& a = pp.args;
+  memset(&a, 0, sizeof(a));
+  pp.elt_in = elt_in;
+  pp.elt_out = elt_out;
+  pp.tw_id = -1;
+  pp.aux_id = -1;
+  pp.variant = 0;
+  pp.in_elems = p.E;
+  pp.out_elems = p.E;
+  for (int d = 0; d < N; ++d) {
+    pp.in_elems *= dims_in[d];
+    pp.out_elems *= dims_out[d];
+  }
+
+  int64_t sin_[kMaxDims], sout[kMaxDims], sin_e, sout_e;
+  layout_strides(N, perm_in, dims_in, sin_, &sin_e);
+  layout_strides(N, perm_out, dims_out, sout, &sout_e);
+
+  a.L_in = dims_in[dim];
+  a.L_out = dims_out[dim];
+  a.sk_in = sin_[dim];
+  a.sk_out = sout[dim];
+  a.in_cplx = eltype_is_complex(elt_in);
+  a.out_cplx = eltype_is_complex(elt_out);
+
+  // line kind, logical length, FFT length
+  const int32_t b = kind_base(kind);
+  int64_t n = a.L_in, M = 0;
+  int lk = L_COPY;
+  int fft_bwd = 0;
+  if (b == PFFT_NOTRANSFORM) {
+    lk = L_COPY;
+  } else if (is_c2c(b)) {
+    lk = L_C2C;
+    M = n;
+    fft_bwd = b == PFFT_BFFT;
+  } else if (b == PFFT_RFFT) {
+    if (n % 2 == 0) { lk = L_R2C_HALF; M = n / 2; }
+    else { lk = L_R2C_FULL; M = n; }
+  } else if (is_c2r(b)) {
+    n = a.L_out;
+    if (n % 2 == 0) { lk = L_C2R_HALF; M = n / 2; }
+    else { lk = L_C2R_FULL; M = n; }
+    fft_bwd = 1;
+  } else {
+    const int32_t k = b - PFFT_R2R;
+    a.r2r_cplx = a.in_cplx;
+    switch (k) {
+      case PFFT_REDFT00:
+        if (n < 2) return fail(PFFT_ERR_ARGUMENT, "REDFT00 requires a length of at least 2");
+        lk = L_DCT1; M = 2 * (n - 1); break;
+      case PFFT_REDFT10: lk = L_DCT2; M = n; break;
+      case PFFT_REDFT01: lk = L_DCT3; M = n; fft_bwd = 1; break;
+      case PFFT_REDFT11: lk = L_DCT4; M = 2 * n; break;
+      case PFFT_RODFT00: lk = L_DST1; M = 2 * (n + 1); break;
+      case PFFT_RODFT10: lk = L_DST2; M = n; break;
+      case PFFT_RODFT01: lk = L_DST3; M = n; fft_bwd = 1; break;
+      case PFFT_RODFT11: lk = L_DST4; M = 2 * n; break;
+      case PFFT_DHT: lk = L_DHT; M = n; break;
+      default: return fail(PFFT_ERR_ARGUMENT, "unsupported r2r kind");
+    }
+  }
+  if (lk == L_COPY) a.r2r_cplx = 0;
+  a.kind = lk;
+  a.bwd = fft_bwd;
+  a.n = n;
+  a.M = M;
+
+  if (lk != L_COPY) {
+    const int ns = radix_schedule(M, a.radix);
+    if (ns < 0) return fail(PFFT_ERR_UNSUPPORTED, "too many radix stages");
+    a.nstages = ns;
+    pp.tw_id = twiddle_table(p, M);
+    // aux tables
+    std::vector<double> aux;
+    if (lk == L_R2C_HALF || lk == L_C2R_HALF) {
+      aux.resize(2 * (M + 1));
+      for (int64_t k = 0; k <= M; ++k) expm(k, n, &aux[2 * k], &aux[2 * k + 1]);
+    } else if (lk == L_DCT2 || lk == L_DST2 || lk == L_DCT3 || lk == L_DST3) {
+      aux.resize(2 * n);
+      for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
+    } else if (lk == L_DCT4 || lk == L_DST4) {
+      aux.resize(4 * n);
+      for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
+      for (int64_t k = 0; k < n; ++k)
+        expm(2 * k + 1, 8 * n, &aux[2 * (n + k)], &aux[2 * (n + k) + 1]);
+    }
+    if (!aux.empty()) pp.aux_id = add_table(p, "aux", std::move(aux));
+  }
+
+  // line dims: every logical dim but `dim`, plus the flattened extra dim
+  struct LD { int64_t len, si, so; int logical; };
+  std::vector<LD> lds;
+  for (int d = 0; d < N; ++d)
+    if (d != dim) lds.push_back({dims_in[d], sin_[d], sout[d], d});
+  if (p.E > 1) lds.push_back({p.E, sin_e, sout_e, N});
+  if (lds.empty()) lds.push_back({1, 0, 0, -1});
+
+  // Line dims in input-memory order; then the tile axis goes first: the
+  // output's contiguous axis when the store transposes, else the input's
+  // contiguous axis when the load is strided, else the line dim nearest in
+  // input memory (consecutive lines are then adjacent in memory).
+  std::stable_sort(lds.begin(), lds.end(), [](const LD& x, const LD& y) { return x.si < y.si; });
+  int tile = -1;
+  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
+    if (a.sk_out != 1 && lds[i].so == 1 && lds[i].len > 1) tile = (int)i;
+  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
+    if (a.sk_in != 1 && lds[i].si == 1 && lds[i].len > 1) tile = (int)i;
+  if (tile > 0) {
+    const LD t = lds[tile];
+    lds.erase(lds.begin() + tile);
+    lds.insert(lds.begin(), t);
+  }
+  // drop unit dims except the first
+  std::vector<LD> keep;
+  for (size_t i = 0; i < lds.size(); ++i)
+    if (i == 0 || lds[i].len > 1) keep.push_back(lds[i]);
+  if ((int)keep.size() > kMaxDims) return fail(PFFT_ERR_UNSUPPORTED, "too many dimensions");
+  a.D = (int32_t)keep.size();
+  for (int i = 0; i < a.D; ++i) {
+    a.ldim[i] = keep[i].len;
+    a.lsin[i] = keep[i].si;
+    a.lsout[i] = keep[i].so;
+  }
+
+  // tile size and LDS footprint of the generic pass
+  const int64_t rs = p.real_type == PFFT_F32 ? 4 : 8;
+  const int64_t cs = 2 * rs;
+  const int comp_in = a.in_cplx ? 2 : 1;
+  a.raw_pitch = comp_in * a.L_in + comp_in;
+  a.Mp = M > 0 ? (M % 2 == 0 ? M + 1 : M) : 0;
+  auto footprint = [&](int64_t T, int64_t* w1off) {
+    const int64_t Tw = a.r2r_cplx ? 2 * T : T;
+    const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
+    const int64_t w1 = (lk == L_COPY || a.nstages == 0) ? 0 : Tw * a.Mp * cs;
+    const int64_t raw = T * a.raw_pitch * rs;
+    *w1off = (w0 + 255) / 256 * 256;
+    return *w1off + std::max(w1, raw);
+  };
+  int64_t T = 1;
+  while (T < 64 && T < a.ldim[0]) T *= 2;
+  int64_t off = 0;
+  int64_t bytes = footprint(T, &off);
+  while (bytes > 64 * 1024 && T > 1) {
+    T /= 2;
+    bytes = footprint(T, &off);
+  }
+  if (bytes > 160 * 1024)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "line of length " + std::to_string(n) + " does not fit the LDS-resident pass");
+  a.T = (int32_t)T;
+  a.lds_w1_off = off;
+  pp.lds_bytes = bytes;
+  pp.threads = kBlockThreads;
+  int64_t nt = (a.ldim[0] + T - 1) / T;
+  for (int i = 1; i < a.D; ++i) nt *= a.ldim[i];
+  a.ntiles = nt;
+  pp.grid = std::min<int64_t>(nt, 1 << 20);
+  a.scale = 1.0;
+  a.do_scale = 0;
+  return PFFT_OK;
+}
+
+// ---- plan construction -----------------------------------------------------
+int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
+               int32_t real_type, int32_t permute, int32_t n_extra, const int64_t* extra,
+               uint32_t flags) {
+  if (N < 1 || N > kMaxDims - 1)
+    return fail(PFFT_ERR_ARGUMENT, "number of transformed dimensions must be in 1.." +
+                                       std::to_string(kMaxDims - 1));
+  if (n_extra < 0 || n_extra > kMaxDims) return fail(PFFT_ERR_ARGUMENT, "too many extra dims");
+  if (real_type != PFFT_F32 && real_type != PFFT_F64)
+    return fail(PFFT_ERR_ARGUMENT, "real type must be Float32 or Float64");
+  p.N = N;
+  p.Ne = n_extra;
+  p.real_type = real_type;
+  p.permute = permute != 0;
+  p.flags = flags;
+  p.E = 1;
+  for (int i = 0; i < n_extra; ++i) {
+    if (extra[i] < 1) return fail(PFFT_ERR_ARGUMENT, "extra dims must be positive");
+    p.extra[i] = extra[i];
+    p.E *= extra[i];
+  }
+  for (int d = 0; d < N; ++d) {
+    std::string why;
+    if (!kind_valid(kinds[d], &why)) return fail(PFFT_ERR_ARGUMENT, why);
+    if (dims[d] < 1) return fail(PFFT_ERR_ARGUMENT, "global dimensions must be positive");
+    p.kinds[d] = kinds[d];
+    p.size_in[d] = dims[d];
+  }
+  // GlobalFFTParams: in-place consistency (global_params.jl:56-60)
+  bool ip0 = kind_inplace(kinds[0]);
+  for (int d = 1; d < N; ++d) {
+    if (kind_inplace(kinds[d]) != ip0) {
+      std::string s = "cannot combine in-place and out-of-place transforms: (";
+      for (int e = 0; e < N; ++e) s += (e ? ", " : "") + kind_name(kinds[e]);
+      s += ")";
+      return fail(PFFT_ERR_ARGUMENT, s);
+    }
+  }
+  p.inplace = ip0;
+  for (int d = 0; d < N; ++d) {
+    int r = length_output(kinds[d], dims[d], &p.size_out[d]);
+    if (r) return r;
+    if (p.size_out[d] < 1) return fail(PFFT_ERR_ARGUMENT, "transform output length is zero");
+  }
+  // input element type (global_params.jl:81-95)
+  int32_t T = real_type;
+  for (int d = 0; d < N; ++d) {
+    int32_t t;
+    int r = eltype_input(kinds[d], real_type, &t);
+    if (r) return r;
+    if (t != PFFT_NOTHING) { T = t; break; }
+  }
+  p.elt_in = T;
+
+  // stages (plans.jl:406-446, 523-553)
+  p.stages.clear();
+  p.scale = 1.0;
+  int32_t Ti = T;
+  for (int n = 0; n < N; ++n) {
+    Stage s;
+    memset(&s, 0, sizeof(s));
+    s.dim = n;
+    s.kind_fwd = kinds[n];
+    int r = eltype_output(kinds[n], Ti, &s.elt_out);
+    if (r) return r;
+    s.elt_in = Ti;
+    chain_perm(N, n, p.permute, s.perm);
+    for (int j = 0; j < N; ++j) {
+      s.dims_before[j] = j < n ? p.size_out[j] : p.size_in[j];
+      s.dims_after[j] = j <= n ? p.size_out[j] : p.size_in[j];
+    }
+    for (int i = 0; i < N; ++i) if (s.perm[i] == n) s.mem_axis = i;
+    s.n_in = p.size_in[n];
+    s.n_out = p.size_out[n];
+    // binv with the input length along the transformed dimension
+    r = binv(kinds[n], s.n_in, &s.kind_bwd);
+    if (r) return r;
+    int64_t sc;
+    r = scale_factor_1d(s.kind_bwd, s.n_out, &sc);
+    if (r) return r;
+    p.scale *= (double)sc;
+    p.stages.push_back(s);
+    Ti = s.elt_out;
+  }
+  p.elt_out = Ti;
+
+  if (p.inplace)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "in-place plans (FFT!, RFFT!, ...) are not provided by this build");
+
+  // passes
+  p.tables.clear();
+  p.fwd.clear();
+  p.bwd.clear();
+  for (int n = 0; n < N; ++n) {
+    const Stage& s = p.stages[n];
+    const Stage& nx = p.stages[n + 1 < N ? n + 1 : n];
+    PassPlan pp;
+    int r = build_pass(p, n, s.kind_fwd, s.elt_in, s.elt_out, s.perm, s.dims_before, nx.perm,
+                       s.dims_after, pp);
+    if (r) return r;
+    p.fwd.push_back(pp);
+  }
+  for (int n = N - 1; n >= 0; --n) {
+    const Stage& s = p.stages[n];
+    const Stage& pv = p.stages[n > 0 ? n - 1 : 0];
+    PassPlan pp;
+    int r = build_pass(p, n, s.kind_bwd, s.elt_out, s.elt_in, s.perm, s.dims_after, pv.perm,
+                       s.dims_before, pp);
+    if (r) return r;
+    p.bwd.push_back(pp);
+  }
+  // buffers: user in -> work0 -> work1 -> work0 ... -> user out
+  size_t w[2] = {0, 0};
+  for (auto* list : {&p.fwd, &p.bwd}) {
+    const int np = (int)list->size();
+    for (int i = 0; i < np; ++i) {
+      PassPlan& pp = (*list)[i];
+      pp.src = i == 0 ? 0 : 1 + ((i - 1) & 1);
+      pp.dst = i == np - 1 ? 0 : 1 + (i & 1);
+      if (pp.dst) {
+        const size_t bytes = (size_t)pp.out_elems * eltype_bytes(pp.elt_out);
+        w[pp.dst - 1] = std::max(w[pp.dst - 1], bytes);
+      }
+    }
+  }
+  w[0] = (w[0] + 255) / 256 * 256;
+  w[1] = (w[1] + 255) / 256 * 256;
+  p.work_bytes = w[0] + w[1];
+  p.work0_bytes = w[0];
+  p.in_bytes = (size_t)p.fwd.front().in_elems * eltype_bytes(p.elt_in);
+  p.out_bytes = (size_t)p.fwd.back().out_elems * eltype_bytes(p.elt_out);
+  return PFFT_OK;
+}
+
+std::string plan_describe(const Plan& p) {
+  // GlobalFFTParams show (global_params.jl:69-75) + extra dims (plans.jl:555-562)
+  std::ostringstream o;
+  o << "Transforms: (";
+  for (int d = 0; d < p.N; ++d) o << (d ? ", " : "") << kind_name(p.kinds[d]);
+  o << (p.N == 1 ? ",)" : ")");
+  o << "\nInput type: " << eltype_name(p.elt_in);
+  auto tup = [&](const int64_t* v, int n) {
+    o << "(";
+    for (int i = 0; i < n; ++i) o << (i ? ", " : "") << v[i];
+    o << (n == 1 ? ",)" : ")");
+  };
+  o << "\nGlobal dimensions: ";
+  tup(p.size_in, p.N);
+  o << " -> ";
+  tup(p.size_out, p.N);
+  if (p.Ne) {
+    o << "\nExtra dimensions: ";
+    tup(p.extra, p.Ne);
+  }
+  o << "\nMPI topology: 1D decomposition (1 processes), one GPU";
+  return o.str();
+}
+
+}  // namespace pfft

@@ -1,0 +1,107 @@
+// Host-side plan of the transform pipeline (the PLAN layer of PencilFFTs:
+// src/plans.jl, src/global_params.jl, src/Transforms/*.jl), and the pass
+// descriptors the execution layer hands to the kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pfft {
+
+// ---- errors ----------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(int code, const std::string& msg);
+int fail(int code, const std::string& msg);  // set_error + return code
+const char* last_error();
+
+// ---- 1D transform metadata (src/Transforms) ---------------------------------
+bool kind_valid(int32_t kind, std::string* why);
+bool kind_inplace(int32_t kind);
+int32_t kind_base(int32_t kind);  // without the in-place bit
+std::string kind_name(int32_t kind);       // Base.show of the transform
+std::string kind_type_name(int32_t kind);  // fully qualified type name
+std::string eltype_name(int32_t eltype);
+int length_output(int32_t kind, int64_t n_in, int64_t* n_out);
+int binv(int32_t kind, int64_t d, int32_t* inv);
+int scale_factor_1d(int32_t kind, int64_t n_in, int64_t* scale);
+int eltype_input(int32_t kind, int32_t real_type, int32_t* out);
+int eltype_output(int32_t kind, int32_t eltype_in, int32_t* out);
+
+// ---- pass (one fused kernel launch over the whole array) --------------------
+struct PassPlan {
+  PassArgs args;            // pointers (in/out/tw/aux) patched at launch
+  int32_t elt_in, elt_out;  // element types of the pass input/output
+  int64_t in_elems, out_elems;
+  int32_t tw_id;            // index into Plan::tables (twiddles), -1 = none
+  int32_t aux_id;           // index into Plan::tables (aux), -1 = none
+  int32_t variant;          // kernel family chosen (0 = generic LDS pass)
+  int32_t threads;
+  int64_t lds_bytes;
+  int64_t grid;
+  int32_t src;              // 0 = user input, 1 = work0, 2 = work1
+  int32_t dst;              // 0 = user output, 1 = work0, 2 = work1
+};
+
+struct Table {
+  int32_t real_bytes;            // 4 or 8
+  std::vector<double> values;    // interleaved re, im (double precision)
+  std::vector<uint8_t> host;     // rounded to the plan's precision
+  void* dev = nullptr;           // device copy
+};
+
+// ---- stage of the pencil chain ----------------------------------------------
+struct Stage {
+  int32_t dim;          // logical dimension (0-based)
+  int32_t kind_fwd, kind_bwd;
+  int64_t n_in, n_out;  // length along `dim`, forward input / output
+  int32_t elt_in, elt_out;
+  int32_t perm[kMaxDims];  // memory order: memory axis i holds logical perm[i]
+  int64_t dims_before[kMaxDims], dims_after[kMaxDims];
+  int32_t mem_axis;        // memory axis of the transformed dimension
+};
+
+struct DeviceState;
+
+struct Plan {
+  int32_t N = 0;             // transformed dims
+  int32_t Ne = 0;            // extra dims
+  int64_t extra[kMaxDims] = {0};
+  int64_t E = 1;             // product of extra dims
+  int64_t size_in[kMaxDims] = {0}, size_out[kMaxDims] = {0};
+  int32_t kinds[kMaxDims] = {0};
+  int32_t real_type = 1;     // PFFT_F32 / PFFT_F64
+  bool permute = true;
+  bool inplace = false;
+  uint32_t flags = 0;
+  double scale = 1.0;
+  int32_t elt_in = 0, elt_out = 0;
+  std::vector<Stage> stages;
+  std::vector<PassPlan> fwd, bwd;  // pass lists per direction
+  std::vector<Table> tables;
+  size_t in_bytes = 0, out_bytes = 0, work_bytes = 0, work0_bytes = 0;
+  DeviceState* dev = nullptr;
+};
+
+int plan_build(Plan& p, int32_t ndims, const int64_t* dims, const int32_t* kinds,
+               int32_t real_type, int32_t permute, int32_t n_extra,
+               const int64_t* extra, uint32_t flags);
+std::string plan_describe(const Plan& p);
+
+// device side (kernels.hip)
+int device_prepare(Plan& p);
+void device_release(Plan& p);
+int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
+                   hipStream_t stream);
+int device_set_timing(Plan& p, bool on);
+int device_stage_times(Plan& p, int n, float* ms);
+size_t eltype_bytes(int32_t eltype);
+
+}  // namespace pfft

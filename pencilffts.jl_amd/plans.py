@@ -1,0 +1,393 @@
+"""PencilFFTPlan and the arrays it transforms, on one GPU.
+
+Mirror of src/plans.jl, src/allocate.jl and src/operations.jl of
+PencilFFTs.jl v0.15.3.  A plan wraps a C-ABI plan (include/pfft.h); arrays are
+torch tensors on the GPU holding the column-major memory image of a
+PencilArray.  Julia's `!` functions are spelled with a trailing underscore:
+mul_ = mul!, ldiv_ = ldiv!, bmul_ = PencilFFTs.bmul!.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import transforms as Tr
+
+_TORCH_DTYPES = {L.F32: torch.float32, L.F64: torch.float64,
+                 L.C32: torch.complex64, L.C64: torch.complex128}
+_NAMES = Tr.ELTYPE_NAMES
+
+
+def _real_type_code(t):
+    if t in (L.F32, L.F64):
+        return t
+    if t in (np.float32, torch.float32, "Float32", "float32"):
+        return L.F32
+    if t in (np.float64, torch.float64, float, "Float64", "float64", None):
+        return L.F64
+    raise L.ArgumentError(L.ERR_ARGUMENT, "real_type must be Float32 or Float64, got %r" % (t,))
+
+
+class Pencil:
+    """Layout of one array of a plan: logical global size, memory permutation
+    (1-based, memory axis i holds logical dim perm[i]) and extra dims."""
+
+    def __init__(self, size_global, permutation, extra_dims=()):
+        self.size_global = tuple(int(s) for s in size_global)
+        self.permutation = tuple(int(p) for p in permutation)
+        self.extra_dims = tuple(int(e) for e in extra_dims)
+
+    def mem_dims(self):
+        return tuple(self.size_global[p - 1] for p in self.permutation) + self.extra_dims
+
+    def __eq__(self, other):
+        return (isinstance(other, Pencil) and self.size_global == other.size_global
+                and self.permutation == other.permutation
+                and self.extra_dims == other.extra_dims)
+
+    def __repr__(self):
+        return "Pencil(size_global=%s, permutation=%s, extra_dims=%s)" % (
+            self.size_global, self.permutation, self.extra_dims)
+
+
+class PencilArray:
+    """A GPU array on a pencil.
+
+    `parent` is a contiguous torch tensor with the memory dims in reverse
+    (torch is row-major), so `parent.reshape(-1)` is the column-major memory
+    image of the Julia array.  `logical()` is a view indexed in logical order
+    (i1, i2, ..., e1, ...), like indexing a PencilArray.
+    """
+
+    def __init__(self, pencil, eltype, parent):
+        self.pencil = pencil
+        self.eltype = eltype
+        self.parent = parent
+
+    @classmethod
+    def empty(cls, pencil, eltype, device="cuda"):
+        dims = pencil.mem_dims()
+        t = torch.empty(tuple(reversed(dims)), dtype=_TORCH_DTYPES[eltype], device=device)
+        return cls(pencil, eltype, t)
+
+    def logical(self):
+        perm = self.pencil.permutation
+        N = len(perm)
+        ne = len(self.pencil.extra_dims)
+        nd = N + ne
+        # torch axis of memory axis i is nd-1-i; logical dim d is memory axis perm.index(d+1)
+        order = [nd - 1 - perm.index(d + 1) for d in range(N)]
+        order += [nd - 1 - (N + e) for e in range(ne)]
+        return self.parent.permute(*order)
+
+    def size_global(self):
+        return self.pencil.size_global
+
+    @property
+    def permutation(self):
+        return self.pencil.permutation
+
+    def data_ptr(self):
+        return self.parent.data_ptr()
+
+    def nbytes(self):
+        return self.parent.numel() * self.parent.element_size()
+
+    def similar(self):
+        return PencilArray(self.pencil, self.eltype, torch.empty_like(self.parent))
+
+    def copy(self):
+        return PencilArray(self.pencil, self.eltype, self.parent.clone())
+
+    def __repr__(self):
+        return "PencilArray{%s}(%s)" % (_NAMES[self.eltype], self.pencil)
+
+
+def _mightalias(a, b):
+    pa, pb = a.data_ptr(), b.data_ptr()
+    return pa < pb + b.nbytes() and pb < pa + a.nbytes()
+
+
+class PencilFFTPlan:
+    """Plan for an N-dimensional transform on one GPU (plans.jl:204-320).
+
+    PencilFFTPlan(dims_global, transforms, proc_dims=(1,), comm=None,
+                  real_type=Float64, extra_dims=(), permute_dims=True, ...)
+    PencilFFTPlan(A::PencilArray, transforms, ...)
+
+    `transforms` is one transform (expanded with expand_dims, plans.jl:316-320)
+    or one per dimension.  proc_dims must describe a single process (one GPU);
+    fftw_flags, fftw_timelimit and transpose_method are accepted and ignored,
+    as the reference does for GPU arrays (plans.jl:325-332).
+    """
+
+    def __init__(self, dims_global, transforms, proc_dims=None, comm=None, real_type=None, *,
+                 extra_dims=(), permute_dims=True, timer=None, fftw_flags=None,
+                 fftw_timelimit=None, transpose_method=None, flags=0, device="cuda"):
+        if isinstance(dims_global, PencilArray):
+            A = dims_global
+            if real_type is None:
+                real_type = L.F32 if A.eltype in (L.F32, L.C32) else L.F64
+            extra_dims = A.pencil.extra_dims
+            if A.pencil.permutation != tuple(range(1, len(A.pencil.size_global) + 1)):
+                raise L.ArgumentError(L.ERR_ARGUMENT,
+                                      "dimensions of input array must be unpermuted")
+            dims_global = A.pencil.size_global
+            input_eltype = A.eltype
+        else:
+            input_eltype = None
+        dims_global = tuple(int(d) for d in dims_global)
+        N = len(dims_global)
+        if proc_dims is not None and int(np.prod(proc_dims)) != 1:
+            raise L.UnsupportedError(L.ERR_UNSUPPORTED,
+                                     "this build runs on one GPU: proc_dims must multiply to 1,"
+                                     " got %s" % (tuple(proc_dims),))
+        if isinstance(transforms, Tr.AbstractTransform):
+            transforms = Tr.expand_dims(transforms, N)
+        transforms = tuple(transforms)
+        if len(transforms) != N:
+            raise L.ArgumentError(L.ERR_ARGUMENT,
+                                  "number of transforms (%d) must be equal to number of "
+                                  "transformable dimensions in array (`ndims(pencil(A))`)"
+                                  % len(transforms))
+        self.transforms = transforms
+        self.real_type = _real_type_code(real_type)
+        self.permute_dims = bool(permute_dims)
+        self.extra_dims = tuple(int(e) for e in extra_dims)
+        self.timer = timer
+        self.device = device
+        h = C.c_void_p()
+        dims_arr = (C.c_int64 * N)(*dims_global)
+        kinds = (C.c_int32 * N)(*[t.code for t in transforms])
+        ne = len(self.extra_dims)
+        ext = (C.c_int64 * max(ne, 1))(*self.extra_dims)
+        L.check(L.lib.pfft_plan_create(C.byref(h), N, dims_arr, kinds, self.real_type,
+                                       int(self.permute_dims), ne, ext, int(flags)))
+        self._h = h
+        self._lib = L.lib
+        self.N = N
+        # queries
+        ei, eo = C.c_int32(), C.c_int32()
+        L.check(L.lib.pfft_plan_eltypes(h, C.byref(ei), C.byref(eo)))
+        self.eltype_input, self.eltype_output = ei.value, eo.value
+        if input_eltype is not None and input_eltype != self.eltype_input:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "wrong input datatype %s, expected %s\n%s" % (
+                _NAMES[input_eltype], _NAMES[self.eltype_input], self.describe()))
+        sc = C.c_double()
+        L.check(L.lib.pfft_plan_scale_factor(h, C.byref(sc)))
+        self.scale_factor = sc.value
+        nd = N + ne
+        sin_, sout = (C.c_int64 * nd)(), (C.c_int64 * nd)()
+        L.check(L.lib.pfft_plan_size_global(h, 0, sin_))
+        L.check(L.lib.pfft_plan_size_global(h, 1, sout))
+        pin, pout = (C.c_int32 * N)(), (C.c_int32 * N)()
+        L.check(L.lib.pfft_plan_permutation(h, 0, pin))
+        L.check(L.lib.pfft_plan_permutation(h, 1, pout))
+        self.size_global_in = tuple(sin_[:N])
+        self.size_global_out = tuple(sout[:N])
+        self.pencil_input = Pencil(self.size_global_in, tuple(pin), self.extra_dims)
+        self.pencil_output = Pencil(self.size_global_out, tuple(pout), self.extra_dims)
+        bi, bo, bw = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        L.check(L.lib.pfft_plan_nbytes(h, C.byref(bi), C.byref(bo), C.byref(bw)))
+        self.in_bytes, self.out_bytes, self.work_bytes = bi.value, bo.value, bw.value
+        self._work = None
+
+    # ---- reference accessors (plans.jl:555-591) ----
+    def size(self):
+        """Global dims of the input (plans.jl:588)."""
+        return self.size_global_in
+
+    def is_inplace(self):
+        return False
+
+    def describe(self):
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib.pfft_plan_describe(self._h, buf, 4096))
+        return buf.value.decode()
+
+    __repr__ = describe
+
+    def stage(self, n):
+        """Pencil chain of 1-based stage n: (perm, dims_before, dims_after, mem_axis)."""
+        N = self.N
+        pi, po = (C.c_int32 * N)(), (C.c_int32 * N)()
+        di, do = (C.c_int64 * N)(), (C.c_int64 * N)()
+        ax = C.c_int32()
+        L.check(L.lib.pfft_plan_stage(self._h, n - 1, pi, di, po, do, C.byref(ax)))
+        return tuple(pi), tuple(di), tuple(do), ax.value
+
+    def kinds(self):
+        N = self.N
+        f, b = (C.c_int32 * N)(), (C.c_int32 * N)()
+        L.check(L.lib.pfft_plan_kinds(self._h, f, b))
+        return tuple(f), tuple(b)
+
+    def set_timing(self, on=True):
+        L.check(L.lib.pfft_plan_set_timing(self._h, int(on)))
+
+    def stage_times(self):
+        n = self.N
+        ms = (C.c_float * n)()
+        L.check(L.lib.pfft_plan_stage_times(self._h, n, ms))
+        return list(ms)
+
+    # ---- execution (operations.jl:24-62) ----
+    def _workspace(self):
+        if self.work_bytes and (self._work is None or self._work.numel() < self.work_bytes):
+            self._work = torch.empty(self.work_bytes, dtype=torch.uint8, device=self.device)
+        return self._work
+
+    def _check_arrays(self, inp, out, inp_pencil, out_pencil, inp_type, out_type):
+        # operations.jl:87-127
+        for a in (inp, out):
+            if not isinstance(a, PencilArray):
+                raise L.ArgumentError(L.ERR_ARGUMENT, "array types (%s, %s) incompatible with "
+                                      "out-of-place plans" % (type(inp).__name__,
+                                                              type(out).__name__))
+        if _mightalias(inp, out):
+            raise L.ArgumentError(L.ERR_ARGUMENT, "out-of-place plan applied to aliased data")
+        if inp.pencil != inp_pencil or inp.eltype != inp_type:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of input data")
+        if out.pencil != out_pencil or out.eltype != out_type:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of output data")
+        for a in (inp, out):
+            if not a.parent.is_cuda or not a.parent.is_contiguous():
+                raise L.ArgumentError(L.ERR_ARGUMENT,
+                                      "arrays must be contiguous GPU tensors")
+
+    def _execute(self, direction, src, dst, stream=None):
+        if direction == L.FORWARD:
+            self._check_arrays(src, dst, self.pencil_input, self.pencil_output,
+                               self.eltype_input, self.eltype_output)
+        else:
+            self._check_arrays(src, dst, self.pencil_output, self.pencil_input,
+                               self.eltype_output, self.eltype_input)
+        w = self._workspace()
+        s = stream if stream is not None else torch.cuda.current_stream(src.parent.device)
+        L.check(L.lib.pfft_execute(self._h, direction, src.data_ptr(), dst.data_ptr(),
+                                   w.data_ptr() if w is not None else None,
+                                   self.work_bytes, s.cuda_stream))
+        return dst
+
+    def __mul__(self, src):
+        """plan * u (operations.jl:54-57)."""
+        if isinstance(src, (tuple, list)):
+            return type(src)(self * u for u in src)
+        dst = allocate_output(self)
+        return mul_(dst, self, src)
+
+    def ldiv(self, src):
+        """plan \\ v (operations.jl:59-62)."""
+        if isinstance(src, (tuple, list)):
+            return type(src)(self.ldiv(v) for v in src)
+        dst = allocate_input(self)
+        return ldiv_(dst, self, src)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.pfft_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def _collection(x):
+    return isinstance(x, (tuple, list))
+
+
+def _check_compatible(a, b):
+    # operations.jl:130-137
+    if len(a) != len(b):
+        raise L.ArgumentError(L.ERR_ARGUMENT, "collections have different lengths: %d ≠ %d"
+                              % (len(a), len(b)))
+
+
+def mul_(dst, p, src):
+    """mul!(dst, p, src): forward transform (operations.jl:24-31)."""
+    if _collection(dst) or _collection(src):
+        _check_compatible(dst, src)
+        for d, s in zip(dst, src):
+            mul_(d, p, s)
+        return dst
+    return p._execute(L.FORWARD, src, dst)
+
+
+def bmul_(dst, p, src):
+    """PencilFFTs.bmul!(dst, p, src): unnormalised backward (operations.jl:34-41)."""
+    if _collection(dst) or _collection(src):
+        _check_compatible(dst, src)
+        for d, s in zip(dst, src):
+            bmul_(d, p, s)
+        return dst
+    return p._execute(L.BACKWARD, src, dst)
+
+
+def ldiv_(dst, p, src):
+    """ldiv!(dst, p, src): normalised backward (operations.jl:44-52)."""
+    if _collection(dst) or _collection(src):
+        _check_compatible(dst, src)
+        for d, s in zip(dst, src):
+            ldiv_(d, p, s)
+        return dst
+    return p._execute(L.BACKWARD_SCALED, src, dst)
+
+
+def allocate_input(p, *dims):
+    """allocate_input(p[, dims...]) / allocate_input(p, Val(N)) with an int
+    count in a 1-tuple (allocate.jl:44-54,85-86)."""
+    if dims:
+        return _allocate_many(allocate_input, p, dims)
+    return PencilArray.empty(p.pencil_input, p.eltype_input, p.device)
+
+
+def allocate_output(p, *dims):
+    """allocate_output(p[, dims...]) (allocate.jl:100-110,115-122)."""
+    if dims:
+        return _allocate_many(allocate_output, p, dims)
+    return PencilArray.empty(p.pencil_output, p.eltype_output, p.device)
+
+
+def _allocate_many(f, p, dims):
+    if len(dims) == 1 and isinstance(dims[0], Val):
+        return tuple(f(p) for _ in range(dims[0].n))
+    n = int(np.prod(dims))
+    arr = np.empty(n, dtype=object)
+    for i in range(n):
+        arr[i] = f(p)
+    return arr.reshape(dims, order="F")
+
+
+class Val:
+    """Val(N): ask allocate_input/allocate_output for an N-tuple."""
+
+    def __init__(self, n):
+        self.n = int(n)
+
+
+def scale_factor(p):
+    return p.scale_factor
+
+
+def execute_many(direction, p, dsts, srcs):
+    """All arrays of a collection in one C call on one stream
+    (pfft_execute_many): the batched form of the collection broadcast."""
+    _check_compatible(dsts, srcs)
+    n = len(srcs)
+    for s, d in zip(srcs, dsts):
+        if direction == L.FORWARD:
+            p._check_arrays(s, d, p.pencil_input, p.pencil_output, p.eltype_input,
+                            p.eltype_output)
+        else:
+            p._check_arrays(s, d, p.pencil_output, p.pencil_input, p.eltype_output,
+                            p.eltype_input)
+    ins = (C.c_void_p * n)(*[s.data_ptr() for s in srcs])
+    outs = (C.c_void_p * n)(*[d.data_ptr() for d in dsts])
+    w = p._workspace()
+    st = torch.cuda.current_stream(srcs[0].parent.device)
+    L.check(L.lib.pfft_execute_many(p._h, direction, n, ins, outs,
+                                    w.data_ptr() if w is not None else None, p.work_bytes,
+                                    st.cuda_stream))
+    return dsts

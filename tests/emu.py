@@ -1,0 +1,55 @@
+"""ctypes wrapper of libpencilffts_emu.so: the host emulator of the generic
+pass (the kernel's phase functions run thread by thread on the CPU).  Used by
+the CPU tests to check the kernel arithmetic and index maps against the
+oracle without a GPU."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "pencilffts.jl_amd", "lib", "libpencilffts_emu.so")
+
+NP_DTYPES = {0: np.float32, 1: np.float64, 2: np.complex64, 3: np.complex128}
+
+
+class Emulator:
+    def __init__(self):
+        if not os.path.exists(PATH):
+            raise RuntimeError("emulator library not built (run __graft_entry__.build())")
+        self.lib = C.CDLL(PATH)
+        L = self.lib
+        L.pfft_emu_plan_create.restype = C.c_int
+        L.pfft_emu_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32,
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                           C.c_int32, C.c_int32, C.c_int32,
+                                           C.POINTER(C.c_int64), C.c_uint32]
+        L.pfft_emu_plan_destroy.argtypes = [C.c_void_p]
+        L.pfft_emu_last_error.restype = C.c_char_p
+        L.pfft_emu_nbytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.pfft_emu_execute.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+
+    def run(self, dims, kinds, real_type, direction, x, permute=True, extra=()):
+        """x: numpy array holding the column-major memory image of the input
+        of `direction`; returns the output memory image (flat)."""
+        N = len(dims)
+        h = C.c_void_p()
+        ne = len(extra)
+        st = self.lib.pfft_emu_plan_create(
+            C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
+            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), 0)
+        if st:
+            raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+        try:
+            bi, bo = C.c_size_t(), C.c_size_t()
+            self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
+            nin, nout = (bi.value, bo.value) if direction == 0 else (bo.value, bi.value)
+            src = np.ascontiguousarray(x).reshape(-1)
+            assert src.nbytes == nin, (src.nbytes, nin)
+            out = np.zeros(nout, dtype=np.uint8)
+            st = self.lib.pfft_emu_execute(h, direction, src.ctypes.data, out.ctypes.data)
+            if st:
+                raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+            return out
+        finally:
+            self.lib.pfft_emu_plan_destroy(h)
