@@ -22,7 +22,7 @@ struct pfft_emu_plan {
 
 template <typename R>
 static void emu_pass(const PassPlan& pp, const PassArgs& a) {
-  std::vector<char> lds((size_t)pp.lds_bytes + 64);
+  std::vector<char> lds((size_t)pp.lds_bytes);  // exact size: overruns show under ASan
   const int nthr = pp.threads;
   for (int64_t tile = 0; tile < a.ntiles; ++tile) {
     TileCtx c;
@@ -74,9 +74,10 @@ __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, in
   Plan& p = h->p;
   if (dir != PFFT_FORWARD && dir != PFFT_BACKWARD && dir != PFFT_BACKWARD_SCALED)
     return fail(PFFT_ERR_ARGUMENT, "unknown direction");
-  std::vector<char> work(p.work_bytes + 64);
+  // the two work buffers as separate exact-size allocations
+  std::vector<char> work0(p.work0_bytes), work1(p.work_bytes - p.work0_bytes);
   const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
-  void* bufs[3] = {nullptr, work.data(), work.data() + p.work0_bytes};
+  void* bufs[3] = {nullptr, work0.data(), work1.data()};
   const int np = (int)passes.size();
   for (int i = 0; i < np; ++i) {
     const PassPlan& pp = passes[i];

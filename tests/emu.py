@@ -8,7 +8,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "pencilffts.jl_amd", "lib", "libpencilffts_emu.so")
+PATH = os.environ.get("PFFT_EMU_LIB") or os.path.join(ROOT, "pencilffts.jl_amd", "lib", "libpencilffts_emu.so")
 
 NP_DTYPES = {0: np.float32, 1: np.float64, 2: np.complex64, 3: np.complex128}
 
