@@ -1,0 +1,202 @@
+"""Benchmark of the headline workload: a 512^3 Float32 RFFT + normalised
+BRFFT pair on one MI355X.
+
+A step is one pair, as in the reference's benchmark_rfft
+(benchmarks/benchmarks.jl:236-245): mul!(v, plan, u) followed by
+ldiv!(u, plan, v), inputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--dims 512]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X datasheet
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dims", type=int, default=512)
+    ap.add_argument("--transform", default="RFFT", choices=["RFFT", "FFT"])
+    ap.add_argument("--real", default="Float32", choices=["Float32", "Float64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--flags", type=int, default=0)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(pass_elems_in, pass_elems_out, bytes_in, bytes_out):
+    return pass_elems_in * bytes_in + pass_elems_out * bytes_out
+
+
+def cpu_baseline(dims, transform, real, reps):
+    """The oracle pipeline (per-stage permuted copy + 1D pocketfft transform,
+    a restatement of the reference's execution layer) timed on the host
+    cores, in the workload's precision, on `reps` full pairs."""
+    import numpy as np
+
+    from oracle import pipeline as OP
+    from oracle import plan as OM
+
+    workers = os.cpu_count() or 1
+    meta = OM.PlanMeta(dims, (transform,), "Float64")
+    rng = np.random.default_rng(42)
+    if transform == "RFFT":
+        x = rng.uniform(-1, 1, size=dims).astype(np.float32 if real == "Float32" else np.float64)
+    else:
+        x = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(
+            np.complex64 if real == "Float32" else np.complex128)
+    xm = np.asfortranarray(x)
+    single = real == "Float32"
+    # warm-up on a small case to load pocketfft plans
+    OP.forward(OM.PlanMeta((16, 16, 16), (transform,)), np.zeros((16, 16, 16)))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        v = OP.forward(meta, xm, method="fast", workers=workers, single=single)
+        OP.backward(meta, v, scaled=True, method="fast", workers=workers, single=single)
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * min(times)
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(ms, 3), "unit": "ms/pair", "cores": workers, "kind": "port",
+            "sample": "%d full %s^3 %s %s pairs (oracle pipeline: permuted copy + scipy.fft per "
+                      "stage, %s, workers=%d), best of %d; host CPU: %s"
+                      % (reps, dims[0], real, transform, "float32" if single else "float64",
+                         workers, reps, cpu)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import __graft_entry__ as ge
+    pf = ge.load_package()
+    T = pf.Transforms
+    dims = (args.dims,) * 3
+    real_type = T.Float32 if args.real == "Float32" else T.Float64
+    tr = T.RFFT() if args.transform == "RFFT" else T.FFT()
+    plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, real_type, flags=args.flags)
+    u = pf.allocate_input(plan)
+    v = pf.allocate_output(plan)
+    g = torch.Generator(device="cuda").manual_seed(42)
+    if args.transform == "RFFT":
+        u.parent.uniform_(-1, 1, generator=g)
+    else:
+        u.parent.copy_(torch.complex(torch.randn(u.parent.shape, generator=g, device="cuda",
+                                                 dtype=torch.float32 if real_type == T.Float32
+                                                 else torch.float64),
+                                     torch.randn(u.parent.shape, generator=g, device="cuda",
+                                                 dtype=torch.float32 if real_type == T.Float32
+                                                 else torch.float64)))
+
+    def pair():
+        pf.mul_(v, plan, u)
+        pf.ldiv_(u, plan, v)
+
+    for _ in range(args.warmup):
+        pair()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        pair()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+
+    # per-pass kernel durations from HIP events recorded by the library on the
+    # launch stream (pfft_plan_set_timing), averaged over a few pairs
+    plan.set_timing(True)
+    nrep = max(3, min(args.steps, 20))
+    fwd_ms = [0.0] * 3
+    bwd_ms = [0.0] * 3
+    for _ in range(nrep):
+        pf.mul_(v, plan, u)
+        for i, t in enumerate(plan.stage_times()):
+            fwd_ms[i] += t / nrep
+        pf.ldiv_(u, plan, v)
+        for i, t in enumerate(plan.stage_times()):
+            bwd_ms[i] += t / nrep
+    plan.set_timing(False)
+
+    rb = 4 if real_type == T.Float32 else 8
+    X, Y, Z = dims
+    if args.transform == "RFFT":
+        R = X * Y * Z * rb
+        Cb = (X // 2 + 1) * Y * Z * 2 * rb
+        fwd_bytes = [R + Cb, 2 * Cb, 2 * Cb]          # pass 1 (r2c), 2, 3
+        bwd_bytes = [2 * Cb, 2 * Cb, Cb + R]          # pass 3, 2, 1 (c2r) in launch order
+    else:
+        Cb = X * Y * Z * 2 * rb
+        fwd_bytes = [2 * Cb] * 3
+        bwd_bytes = [2 * Cb] * 3
+    pair_bytes = sum(fwd_bytes) + sum(bwd_bytes)
+    launches = [(fwd_ms[i], fwd_bytes[i], "fwd%d" % (i + 1)) for i in range(3)]
+    launches += [(bwd_ms[i], bwd_bytes[i], "bwd%d" % (3 - i)) for i in range(3)]
+    # the dominant kernel: all six passes run the same kernel family here, so
+    # the roofline is quoted for the family over the pair (bytes / time summed)
+    kern_ms = sum(t for t, _, _ in launches) / len(launches)
+    kern_bytes = sum(b for _, b, _ in launches) / len(launches)
+    achieved = kern_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%d.json" % (args.transform.lower(), X))
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "pass kernels (6 launches per pair)",
+                "bytes_per_launch": kern_bytes, "launch_ms": round(kern_ms, 4),
+                "passes_ms": {name: round(t, 4) for t, _, name in launches}}
+    out = {
+        "metric": "512³ r2c+c2r FFT: ms/pair, % HBM peak; split-FP16 MFMA vs FP32 at 1e-6 rel",
+        "value": round(ms, 4), "unit": "ms/pair", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if real_type == T.Float32 else "f64", "data": "synthetic",
+        "config": {"workload": "%d^3 %s %s+%s pair (mul! + ldiv!)"
+                               % (X, args.real, args.transform,
+                                  "BRFFT" if args.transform == "RFFT" else "BFFT"),
+                   "dims": list(dims), "transforms": args.transform, "real_type": args.real,
+                   "input": "uniform[-1,1) seed 42" if args.transform == "RFFT"
+                            else "re/im iid N(0,1) seed 42",
+                   "parallelism": "single GPU"},
+        "pair_hbm_frac": round(pair_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pair_algorithmic_bytes": pair_bytes,
+        "pairs_per_s": round(1e3 / ms, 2),
+        "wall_s": round(wall, 3),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(dims, args.transform, args.real, args.cpu_reps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,11 +56,17 @@ def _transform(tr, a_mem, axis, method, workers):
     return T1.apply(tr, a_mem, axis=axis, method=method, workers=workers)
 
 
-def forward(meta, u_mem, method="auto", workers=1):
+def _widen(x, single):
+    if single:
+        return x.astype(np.complex64 if np.iscomplexobj(x) else np.float32, copy=False)
+    return x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+
+
+def forward(meta, u_mem, method="auto", workers=1, single=False):
     """mul!(v, p, u): u_mem in the input pencil's memory order (unpermuted);
-    returns v in the output pencil's memory order, float64/complex128."""
-    x = np.asarray(u_mem)
-    x = x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+    returns v in the output pencil's memory order, float64/complex128
+    (float32/complex64 with single=True, used only to time the CPU baseline)."""
+    x = _widen(np.asarray(u_mem), single)
     perm_prev = meta.stages[0]["perm"]
     for st in meta.stages:
         if st["perm"] != perm_prev:
@@ -70,11 +76,10 @@ def forward(meta, u_mem, method="auto", workers=1):
     return x
 
 
-def backward(meta, v_mem, scaled=True, method="auto", workers=1):
+def backward(meta, v_mem, scaled=True, method="auto", workers=1, single=False):
     """ldiv!(u, p, v) (scaled) or bmul!(u, p, v): v_mem in the output pencil's
     memory order; returns u in the input pencil's memory order."""
-    x = np.asarray(v_mem)
-    x = x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+    x = _widen(np.asarray(v_mem), single)
     perm_prev = meta.stages[-1]["perm"]
     for st in reversed(meta.stages):
         if st["perm"] != perm_prev:
@@ -83,7 +88,7 @@ def backward(meta, v_mem, scaled=True, method="auto", workers=1):
         x = _transform(st["tr_bwd"], x, st["mem_axis"], method, workers)
     # the result lives in stage 1's pencil, which is the input pencil
     if scaled:
-        x = x * (1.0 / meta.scale)
+        x *= x.dtype.type(1.0 / meta.scale) if single else 1.0 / meta.scale
     return x
 
 
