@@ -167,11 +167,13 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    kernels = plan.kernels(0) + plan.kernels(2)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "pass kernels (6 launches per pair)",
                 "bytes_per_launch": kern_bytes, "launch_ms": round(kern_ms, 4),
-                "passes_ms": {name: round(t, 4) for t, _, name in launches}}
+                "passes_ms": {name: round(t, 4) for t, _, name in launches},
+                "passes_kernels": kernels}
     out = {
         "metric": "512³ r2c+c2r FFT: ms/pair, % HBM peak; split-FP16 MFMA vs FP32 at 1e-6 rel",
         "value": round(ms, 4), "unit": "ms/pair", "n_gpus": 1, "steps": args.steps,

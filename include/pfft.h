@@ -166,6 +166,9 @@ int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
  * mirror of the reference's TimerOutputs sections "FFT" per stage
  * (operations.jl:250).  Enabled with pfft_plan_set_timing(plan, 1).        */
 int pfft_plan_set_timing(pfft_plan* plan, int32_t enable);
+/* Kernel chosen for each pass of a direction, ';'-separated in launch order
+ * (prepares the plan on the current device). */
+int pfft_plan_kernels(pfft_plan* plan, int32_t direction, char* buf, size_t buflen);
 int pfft_plan_stage_times(pfft_plan* plan, int32_t n, float* ms);
 
 #ifdef __cplusplus

@@ -48,6 +48,7 @@ SIGNATURES = {
     "pfft_execute": (C.c_int, [vp, i32, vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
     "pfft_plan_set_timing": (C.c_int, [vp, i32]),
+    "pfft_plan_kernels": (C.c_int, [vp, i32, C.c_char_p, sz]),
     "pfft_plan_stage_times": (C.c_int, [vp, i32, P(C.c_float)]),
 }
 

@@ -267,6 +267,19 @@ int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
   })
 }
 
+int pfft_plan_kernels(pfft_plan* plan, int32_t direction, char* buf, size_t buflen) {
+  NEED(plan);
+  NEED(buf);
+  PFFT_GUARD({
+    std::string s;
+    int r = device_pass_names(plan->p, direction, &s);
+    if (r) return r;
+    if (buflen < s.size() + 1) return fail(PFFT_ERR_ARGUMENT, "buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return PFFT_OK;
+  })
+}
+
 int pfft_plan_set_timing(pfft_plan* plan, int32_t enable) {
   NEED(plan);
   PFFT_GUARD(return device_set_timing(plan->p, enable != 0);)

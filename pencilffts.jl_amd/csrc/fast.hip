@@ -1,27 +1,550 @@
-// Specialised register-resident passes for the hot sizes.
+// Specialised passes for power-of-two line lengths (64 ... 1024 complex
+// points per FFT): the hot path of the pipeline.
+//
+// Work split: one line is owned by L lanes of one wavefront (L = M / P), each
+// lane holding P points in registers in the strided order x[l + L p].  The
+// Stockham stages run radix-8/16/4 butterflies in registers; between stages
+// the line goes through a padded LDS region private to the line, ordered by
+// wavefront-scope fences (no workgroup barrier).  The first stage takes its
+// inputs straight from global memory (each load instruction of a wave reads
+// consecutive elements) and the last stage leaves its outputs in registers in
+// the same strided order, so a contiguous line store is coalesced as well.
+//
+// A workgroup owns T lines.  When the output (or input) of the pass is
+// transposed -- the fused PencilArrays transpose of the reference -- the T
+// lines meet in an LDS tile and are written (read) as runs of T consecutive
+// elements with 16-byte accesses.
+//
+// Kinds: complex FFT (the backward sign as conj(FFT(conj x))), r2c and c2r of
+// an even real length n via the n/2-point complex FFT with the split/merge
+// twiddle w_n^k fused in.
 #include <hip/hip_runtime.h>
 
 #include <string>
 
 #include "../../include/pfft.h"
 #include "fast_pass.hpp"
+#include "generic_pass.hpp"
 
 namespace pfft {
+namespace fk {
+
+template <typename R>
+using C = cpx<R>;
+
+// one padding complex every 8 spreads the radix-8 scatter over the banks
+__device__ __forceinline__ constexpr int pad8(int q) { return q + (q >> 3); }
+template <int M>
+constexpr int line_pitch() { return M + (M >> 3) + 2; }  // holds indices 0..M
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- forward small DFTs in registers ----------------------------------------
+template <typename R>
+__device__ __forceinline__ void dft16(C<R>* v) {
+  // 16 = 4 x 4: n = 4a + b, k = k1 + 4 k2
+  const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
+  const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
+  const R h = R(0.70710678118654752440084436210485);
+  C<R> y[16];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    C<R> t[4] = {v[b], v[4 + b], v[8 + b], v[12 + b]};
+    dft4(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[4 * b + k1] = t[k1];
+  }
+  // twiddles w16^(b k1)
+  y[5] = mk(y[5].re * c1 + y[5].im * s1, y[5].im * c1 - y[5].re * s1);      // w^1
+  y[6] = mk((y[6].re + y[6].im) * h, (y[6].im - y[6].re) * h);              // w^2
+  y[7] = mk(y[7].re * s1 + y[7].im * c1, y[7].im * s1 - y[7].re * c1);      // w^3
+  y[9] = mk((y[9].re + y[9].im) * h, (y[9].im - y[9].re) * h);              // w^2
+  y[10] = mul_mi(y[10]);                                                    // w^4
+  y[11] = mk((y[11].im - y[11].re) * h, -(y[11].re + y[11].im) * h);        // w^6
+  y[13] = mk(y[13].re * s1 + y[13].im * c1, y[13].im * s1 - y[13].re * c1); // w^3
+  y[14] = mk((y[14].im - y[14].re) * h, -(y[14].re + y[14].im) * h);        // w^6
+  y[15] = mk(-y[15].re * c1 - y[15].im * s1, -y[15].im * c1 + y[15].re * s1); // w^9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    C<R> t[4] = {y[k1], y[4 + k1], y[8 + k1], y[12 + k1]};
+    dft4(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <typename R, int RAD>
+__device__ __forceinline__ void dft(C<R>* v) {
+  if constexpr (RAD == 2) dft2(v);
+  else if constexpr (RAD == 4) dft4(v);
+  else if constexpr (RAD == 8) dft8(v);
+  else if constexpr (RAD == 16) dft16(v);
+}
+
+// ---- one Stockham stage over one line ------------------------------------------
+// Butterfly j (0 <= j < M/RAD) of a stage with Ns = product of earlier radices:
+//   inputs  x[j + (M/RAD) r] * w_M^(r (j mod Ns) M/(Ns RAD))
+//   outputs y[(j / Ns) Ns RAD + (j mod Ns) + r Ns]
+// Lane l owns butterflies j = l + L i.  With FROM_REG the inputs come from
+// the strided registers v (index i + BPL r); with TO_REG (last stage, Ns RAD
+// == M) the outputs land in v in the same strided order.
+template <typename R, int M, int L, int P, int RAD, int NS, bool FROM_REG, bool TO_REG>
+__device__ __forceinline__ void stage(C<R>* v, C<R>* line, int l, const C<R>* __restrict__ tw) {
+  constexpr int NB = M / RAD;
+  constexpr int BPL = NB / L;
+  static_assert(BPL * RAD == P, "radix must divide the points per lane");
+  static_assert(!TO_REG || NS * RAD == M, "registers hold the output of the last stage only");
+  C<R> u[P];
+#pragma unroll
+  for (int i = 0; i < BPL; ++i)
+#pragma unroll
+    for (int r = 0; r < RAD; ++r)
+      u[i * RAD + r] = FROM_REG ? v[i + BPL * r] : line[pad8(l + L * i + NB * r)];
+#pragma unroll
+  for (int i = 0; i < BPL; ++i) {
+    C<R>* w = u + i * RAD;
+    if constexpr (NS > 1) {
+      const int jj = (l + L * i) % NS;
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) w[r] = w[r] * tw[r * jj * (M / (NS * RAD))];
+    }
+    dft<R, RAD>(w);
+  }
+#pragma unroll
+  for (int i = 0; i < BPL; ++i) {
+    if constexpr (TO_REG) {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[i + BPL * r] = u[i * RAD + r];
+    } else {
+      const int j = l + L * i;
+      const int base = (j / NS) * NS * RAD + (j % NS);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) line[pad8(base + r * NS)] = u[i * RAD + r];
+    }
+  }
+}
+
+template <typename R, int M, int L, int P, int NS, bool FIRST, int RAD, int... REST>
+__device__ __forceinline__ void fft_rec(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+  constexpr bool LAST = sizeof...(REST) == 0;
+  stage<R, M, L, P, RAD, NS, FIRST, LAST>(v, line, l, tw);
+  if constexpr (!LAST) {
+    wave_sync();
+    fft_rec<R, M, L, P, NS * RAD, false, REST...>(v, line, l, tw);
+  }
+}
+
+// radix schedules: M = L * P, every radix divides P
+template <int M> struct Cfg;
+template <> struct Cfg<64> {
+  static constexpr int L = 8, P = 8;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 64, L, P, 1, true, 8, 8>(v, line, l, tw);
+  }
+};
+template <> struct Cfg<128> {
+  static constexpr int L = 16, P = 8;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 128, L, P, 1, true, 8, 4, 4>(v, line, l, tw);
+  }
+};
+template <> struct Cfg<256> {
+  static constexpr int L = 32, P = 8;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 256, L, P, 1, true, 8, 8, 4>(v, line, l, tw);
+  }
+};
+template <> struct Cfg<512> {
+  static constexpr int L = 64, P = 8;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 512, L, P, 1, true, 8, 8, 8>(v, line, l, tw);
+  }
+};
+template <> struct Cfg<1024> {
+  static constexpr int L = 64, P = 16;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 1024, L, P, 1, true, 16, 16, 4>(v, line, l, tw);
+  }
+};
+
+enum { K_C2C = 0, K_R2C = 1, K_C2R = 2 };
+enum { IO_LINE = 0, IO_TILE = 1 };
+
+// tile element (row k, line t): XOR swizzle keeps each row's 16-byte pairs
+// adjacent (float) while spreading the column writes over the banks
+template <typename R, int T>
+__device__ __forceinline__ int tile_idx(int k, int t) {
+  if constexpr (sizeof(R) == 4) return k * T + (t ^ ((k & (T / 2 - 1)) << 1));
+  else return k * T + (t ^ (k & (T - 1)));
+}
+
+template <typename R, int T>
+__device__ __forceinline__ void tile_store(C<R>* tile, int rows, const PassArgs& a, int64_t base,
+                                           int tid, int nthr) {
+  C<R>* out = (C<R>*)a.out;
+  if constexpr (sizeof(R) == 4) {
+    constexpr int HP = T / 2;  // float4 pairs per row
+    const int total = rows * HP;
+    for (int e = tid; e < total; e += nthr) {
+      const int k = e / HP;
+      const int t = 2 * (e - k * HP);
+      const float4 val = *(const float4*)&tile[tile_idx<R, T>(k, t)];
+      *(float4*)&out[base + (int64_t)k * a.sk_out + t] = val;
+    }
+  } else {
+    const int total = rows * T;
+    for (int e = tid; e < total; e += nthr) {
+      const int k = e / T;
+      const int t = e - k * T;
+      out[base + (int64_t)k * a.sk_out + t] = tile[tile_idx<R, T>(k, t)];
+    }
+  }
+}
+
+template <typename R, int T>
+__device__ __forceinline__ void tile_load(C<R>* tile, int rows, const PassArgs& a, int64_t base,
+                                          int tid, int nthr) {
+  const C<R>* in = (const C<R>*)a.in;
+  if constexpr (sizeof(R) == 4) {
+    constexpr int HP = T / 2;
+    const int total = rows * HP;
+    for (int e = tid; e < total; e += nthr) {
+      const int k = e / HP;
+      const int t = 2 * (e - k * HP);
+      *(float4*)&tile[tile_idx<R, T>(k, t)] =
+          *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
+    }
+  } else {
+    const int total = rows * T;
+    for (int e = tid; e < total; e += nthr) {
+      const int k = e / T;
+      const int t = e - k * T;
+      tile[tile_idx<R, T>(k, t)] = in[base + (int64_t)k * a.sk_in + t];
+    }
+  }
+}
+
+template <typename R, int M, int T, int KIND, int LD, int ST>
+constexpr size_t lds_bytes() {
+  constexpr size_t lines = (size_t)T * line_pitch<M>() * sizeof(C<R>);
+  constexpr int rows = KIND == K_C2C ? M : M + 1;
+  constexpr size_t tile = (LD == IO_TILE || ST == IO_TILE) ? (size_t)rows * T * sizeof(C<R>) : 0;
+  return lines > tile ? lines : tile;
+}
+
+template <typename R, int M, int T, int KIND, int LD, int ST>
+__global__ __launch_bounds__(T * Cfg<M>::L) void k_fast(PassArgs a) {
+  constexpr int L = Cfg<M>::L, P = Cfg<M>::P;
+  constexpr int NTHR = T * L;
+  extern __shared__ __align__(16) char smem[];
+  C<R>* lds = (C<R>*)smem;
+  const int tid = threadIdx.x;
+  const int t = tid / L;
+  const int l = tid - t * L;
+  C<R>* line = lds + t * line_pitch<M>();
+  C<R>* tile = lds;
+  const C<R>* __restrict__ tw = (const C<R>*)a.tw;
+  const C<R>* __restrict__ aux = (const C<R>*)a.aux;
+  const bool conj_io = KIND == K_C2C ? a.bwd != 0 : KIND == K_C2R;
+  const R sc = a.do_scale ? (R)a.scale : R(1);
+
+  for (int64_t tl = blockIdx.x; tl < a.ntiles; tl += gridDim.x) {
+    TileCtx c;
+    tile_init(a, tl, c);
+    const bool active = t < c.Tact;
+    C<R> v[P];
+
+    // ---- load ----
+    if constexpr (KIND == K_C2R) {
+      if constexpr (LD == IO_TILE) {
+        tile_load<R, T>(tile, M + 1, a, c.in_base, tid, NTHR);
+        __syncthreads();
+      } else {
+        const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+        if (active) {
+          for (int k = l; k <= M; k += L) line[pad8(k)] = in[k];
+        }
+        wave_sync();
+      }
+      // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int k = l + L * p;
+        C<R> A, B;
+        if constexpr (LD == IO_TILE) {
+          A = tile[tile_idx<R, T>(k, t)];
+          B = tile[tile_idx<R, T>(M - k, t)];
+        } else {
+          A = line[pad8(k)];
+          B = line[pad8(M - k)];
+        }
+        B = conj(B);
+        if (k == 0) { A.im = R(0); B.im = R(0); }
+        const C<R> E = A + B;
+        const C<R> O = (A - B) * conj(aux[k]);
+        v[p] = conj(E + mul_pi(O));   // conj: backward FFT as conj(FFT(conj))
+      }
+      if constexpr (LD == IO_TILE) __syncthreads();
+      else wave_sync();
+    } else if constexpr (LD == IO_TILE) {
+      tile_load<R, T>(tile, M, a, c.in_base, tid, NTHR);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const C<R> x = tile[tile_idx<R, T>(l + L * p, t)];
+        v[p] = conj_io ? conj(x) : x;
+      }
+      __syncthreads();
+    } else {
+      // contiguous line: complex elements, or real pairs (x[2m], x[2m+1]) for r2c
+      const C<R>* in = KIND == K_R2C
+                           ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
+                           : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+      if (active) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const C<R> x = in[l + L * p];
+          v[p] = conj_io ? conj(x) : x;
+        }
+      }
+    }
+
+    // ---- FFT ----
+    Cfg<M>::template fft<R>(v, line, l, tw);
+
+    // ---- post + store ----
+    if constexpr (KIND == K_R2C) {
+      // split: X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
+      wave_sync();
+#pragma unroll
+      for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[p];
+      wave_sync();
+      C<R> X[P + 1];
+#pragma unroll
+      for (int p = 0; p <= P; ++p) {
+        const int k = p < P ? l + L * p : M;
+        const C<R> Z = line[pad8(k & (M - 1))];
+        const C<R> Zc = conj(line[pad8((M - k) & (M - 1))]);
+        const C<R> E = scl(Z + Zc, R(0.5));
+        const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
+        X[p] = E + aux[k] * O;
+      }
+      if constexpr (ST == IO_TILE) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = X[p];
+        if (l == 0) tile[tile_idx<R, T>(M, t)] = X[P];
+        __syncthreads();
+        tile_store<R, T>(tile, M + 1, a, c.out_base, tid, NTHR);
+      } else {
+        C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+        if (active) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) out[l + L * p] = X[p];
+          if (l == 0) out[M] = X[P];
+        }
+      }
+    } else if constexpr (KIND == K_C2R) {
+      // z_m = conj(FFT) -> real pairs x[2m] = Re z_m, x[2m+1] = Im z_m
+      C<R>* out = (C<R>*)((R*)a.out + c.out_base + (int64_t)t * a.lsout[0]);
+      if (active) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) out[l + L * p] = scl(conj(v[p]), sc);
+      }
+    } else {
+      if constexpr (ST == IO_TILE) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const C<R> y = conj_io ? conj(v[p]) : v[p];
+          tile[tile_idx<R, T>(l + L * p, t)] = scl(y, sc);
+        }
+        __syncthreads();
+        tile_store<R, T>(tile, M, a, c.out_base, tid, NTHR);
+      } else {
+        C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+        if (active) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const C<R> y = conj_io ? conj(v[p]) : v[p];
+            out[l + L * p] = scl(y, sc);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- registry --------------------------------------------------------------------
+struct Entry {
+  int real_bytes, M, kind, ld, st, T, threads;
+  size_t lds;
+  const void* fn;
+  void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
+};
+
+template <typename R, int M, int T, int KIND, int LD, int ST>
+static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
+  hipLaunchKernelGGL((k_fast<R, M, T, KIND, LD, ST>), g, b, lds, s, a);
+}
+
+template <typename R, int M, int T, int KIND, int LD, int ST>
+static Entry make() {
+  Entry e;
+  e.real_bytes = sizeof(R);
+  e.M = M;
+  e.kind = KIND;
+  e.ld = LD;
+  e.st = ST;
+  e.T = T;
+  e.threads = T * Cfg<M>::L;
+  e.lds = lds_bytes<R, M, T, KIND, LD, ST>();
+  e.fn = (const void*)k_fast<R, M, T, KIND, LD, ST>;
+  e.launch = launch_one<R, M, T, KIND, LD, ST>;
+  return e;
+}
+
+// lines per workgroup: 16 (f32) / 8 (f64) for transposed tiles (128-byte
+// runs); 256 threads per workgroup for contiguous passes
+template <typename R, int M>
+constexpr int t_tile() { return (sizeof(R) == 4 && M < 1024) ? 16 : 8; }
+template <int M>
+constexpr int t_line() { return 256 / Cfg<M>::L; }
+
+template <typename R, int M>
+static void add_size(std::vector<Entry>& v) {
+  v.push_back(make<R, M, t_line<M>(), K_C2C, IO_LINE, IO_LINE>());
+  v.push_back(make<R, M, t_tile<R, M>(), K_C2C, IO_LINE, IO_TILE>());
+  v.push_back(make<R, M, t_tile<R, M>(), K_C2C, IO_TILE, IO_LINE>());
+  v.push_back(make<R, M, t_line<M>(), K_R2C, IO_LINE, IO_LINE>());
+  v.push_back(make<R, M, t_tile<R, M>(), K_R2C, IO_LINE, IO_TILE>());
+  v.push_back(make<R, M, t_line<M>(), K_C2R, IO_LINE, IO_LINE>());
+  v.push_back(make<R, M, t_tile<R, M>(), K_C2R, IO_TILE, IO_LINE>());
+}
+
+static const std::vector<Entry>& registry() {
+  static std::vector<Entry> v = [] {
+    std::vector<Entry> r;
+    add_size<float, 64>(r);
+    add_size<float, 128>(r);
+    add_size<float, 256>(r);
+    add_size<float, 512>(r);
+    add_size<float, 1024>(r);
+    add_size<double, 64>(r);
+    add_size<double, 128>(r);
+    add_size<double, 256>(r);
+    add_size<double, 512>(r);
+    return r;
+  }();
+  return v;
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+static bool all_even(const int64_t* s, int from, int D) {
+  for (int d = from; d < D; ++d)
+    if (s[d] & 1) return false;
+  return true;
+}
 
 void fast_select(const Plan& p, PassPlan& pp) {
-  (void)p;
-  (void)pp;
+  if (p.flags & PFFT_FLAG_GENERIC_ONLY) return;
+  PassArgs& a = pp.args;
+  int kind;
+  if (a.kind == L_C2C) kind = K_C2C;
+  else if (a.kind == L_R2C_HALF) kind = K_R2C;
+  else if (a.kind == L_C2R_HALF) kind = K_C2R;
+  else return;
+  const int rb = p.real_type == PFFT_F32 ? 4 : 8;
+  const int M = (int)a.M;
+  int ld, st;
+  if (a.sk_in == 1) ld = IO_LINE;
+  else if (a.lsin[0] == 1 && kind != K_R2C) ld = IO_TILE;
+  else return;
+  if (a.sk_out == 1) st = IO_LINE;
+  else if (a.lsout[0] == 1 && kind != K_C2R) st = IO_TILE;
+  else return;
+  if (ld == IO_TILE && st == IO_TILE) return;
+  const std::vector<Entry>& reg = registry();
+  const Entry* e = nullptr;
+  for (const Entry& x : reg)
+    if (x.real_bytes == rb && x.M == M && x.kind == kind && x.ld == ld && x.st == st) e = &x;
+  if (!e) return;
+  if (e->lds > 160 * 1024) return;
+  // transposed tiles: full tiles, 16-byte aligned runs
+  if (ld == IO_TILE || st == IO_TILE) {
+    if (a.ldim[0] % e->T) return;
+    const int64_t sk = ld == IO_TILE ? a.sk_in : a.sk_out;
+    const int64_t* ls = ld == IO_TILE ? a.lsin : a.lsout;
+    if (rb == 4 && ((sk & 1) || !all_even(ls, 1, a.D))) return;
+  }
+  // real pairs of r2c input / c2r output need even line offsets
+  if (kind == K_R2C && !all_even(a.lsin, 0, a.D)) return;
+  if (kind == K_C2R && !all_even(a.lsout, 0, a.D)) return;
+  // contiguous passes: put a line dim that the tile size divides first
+  if (ld == IO_LINE && st == IO_LINE && a.ldim[0] % e->T) {
+    for (int d = 1; d < a.D; ++d) {
+      if (a.ldim[d] % e->T == 0) {
+        std::swap(a.ldim[0], a.ldim[d]);
+        std::swap(a.lsin[0], a.lsin[d]);
+        std::swap(a.lsout[0], a.lsout[d]);
+        break;
+      }
+    }
+  }
+  a.T = e->T;
+  int64_t nt = (a.ldim[0] + e->T - 1) / e->T;
+  for (int d = 1; d < a.D; ++d) nt *= a.ldim[d];
+  a.ntiles = nt;
+  pp.grid = std::min<int64_t>(nt, 1 << 20);
+  pp.threads = e->threads;
+  pp.lds_bytes = (int64_t)e->lds;
+  pp.variant = 1 + (int)(e - reg.data());
 }
 
 int fast_prepare(PassPlan& pp) {
-  (void)pp;
+  const Entry& e = registry()[pp.variant - 1];
+  if (e.lds > 64 * 1024) {
+    hipError_t r = hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)e.lds);
+    if (r != hipSuccess)
+      return fail(PFFT_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(r));
+  }
   return PFFT_OK;
 }
 
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream) {
-  (void)a;
-  (void)stream;
-  return fail(PFFT_ERR_UNSUPPORTED, "no specialised kernel variant " + std::to_string(pp.variant));
+  const std::vector<Entry>& reg = registry();
+  if (pp.variant < 1 || pp.variant > (int)reg.size())
+    return fail(PFFT_ERR_UNSUPPORTED, "no specialised kernel variant " +
+                                          std::to_string(pp.variant));
+  const Entry& e = reg[pp.variant - 1];
+  e.launch(dim3((unsigned)pp.grid), dim3(e.threads), e.lds, stream, a);
+  return PFFT_OK;
+}
+
+const char* fast_name(int variant) {
+  static thread_local std::string s;
+  const std::vector<Entry>& reg = registry();
+  if (variant < 1 || variant > (int)reg.size()) return "generic";
+  const Entry& e = reg[variant - 1];
+  static const char* kinds[3] = {"c2c", "r2c", "c2r"};
+  s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
+      std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
+      (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ">";
+  return s.c_str();
 }
 
 }  // namespace pfft

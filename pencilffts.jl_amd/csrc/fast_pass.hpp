@@ -10,5 +10,6 @@ namespace pfft {
 void fast_select(const Plan& p, PassPlan& pp);
 int fast_prepare(PassPlan& pp);
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream);
+const char* fast_name(int variant);  // kernel name of a variant ("generic" for 0)
 
 }  // namespace pfft

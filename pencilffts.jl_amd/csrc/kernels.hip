@@ -78,6 +78,7 @@ int device_prepare(Plan& p) {
   }
   for (auto* list : {&p.fwd, &p.bwd}) {
     for (PassPlan& pp : *list) {
+      fast_select(p, pp);
       int r = PFFT_OK;
       if (pp.variant == 0) {
         r = p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
@@ -92,6 +93,20 @@ int device_prepare(Plan& p) {
     }
   }
   p.dev = d;
+  return PFFT_OK;
+}
+
+int device_pass_names(Plan& p, int dir, std::string* out) {
+  int r = device_prepare(p);
+  if (r) return r;
+  const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+  out->clear();
+  for (size_t i = 0; i < passes.size(); ++i) {
+    if (i) *out += ";";
+    *out += passes[i].variant == 0
+                ? std::string(p.real_type == PFFT_F32 ? "k_generic<f32>" : "k_generic<f64>")
+                : std::string(fast_name(passes[i].variant));
+  }
   return PFFT_OK;
 }
 

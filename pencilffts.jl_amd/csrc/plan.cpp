@@ -584,23 +584,50 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
   p.tables.clear();
   p.fwd.clear();
   p.bwd.clear();
-  for (int n = 0; n < N; ++n) {
-    const Stage& s = p.stages[n];
-    const Stage& nx = p.stages[n + 1 < N ? n + 1 : n];
-    PassPlan pp;
-    int r = build_pass(p, n, s.kind_fwd, s.elt_in, s.elt_out, s.perm, s.dims_before, nx.perm,
-                       s.dims_after, pp);
-    if (r) return r;
-    p.fwd.push_back(pp);
+  // Intermediate buffers are internal, so their memory order is free.  The
+  // default is the reference chain (pass n writes the pencil of stage n+1);
+  // when that layout would make an odd-length axis the fastest one (e.g. the
+  // n/2+1 axis after an r2c), the store's runs would be misaligned, and the
+  // pass keeps its input's order instead (the next pass then loads strided).
+  auto choose = [&](const int32_t* perm_in, const int32_t* perm_def, const int64_t* dims,
+                    int32_t* perm_out) {
+    memcpy(perm_out, perm_def, sizeof(int32_t) * N);
+    const int64_t fast_len = dims[perm_def[0]];
+    if (fast_len > 1 && (fast_len & 1) &&
+        memcmp(perm_in, perm_def, sizeof(int32_t) * N) != 0)
+      memcpy(perm_out, perm_in, sizeof(int32_t) * N);
+  };
+  {
+    int32_t cur[kMaxDims];
+    memcpy(cur, p.stages[0].perm, sizeof(cur));
+    for (int n = 0; n < N; ++n) {
+      const Stage& s = p.stages[n];
+      int32_t outp[kMaxDims];
+      if (n == N - 1) memcpy(outp, s.perm, sizeof(outp));
+      else choose(cur, p.stages[n + 1].perm, s.dims_after, outp);
+      PassPlan pp;
+      int r = build_pass(p, n, s.kind_fwd, s.elt_in, s.elt_out, cur, s.dims_before, outp,
+                         s.dims_after, pp);
+      if (r) return r;
+      p.fwd.push_back(pp);
+      memcpy(cur, outp, sizeof(cur));
+    }
   }
-  for (int n = N - 1; n >= 0; --n) {
-    const Stage& s = p.stages[n];
-    const Stage& pv = p.stages[n > 0 ? n - 1 : 0];
-    PassPlan pp;
-    int r = build_pass(p, n, s.kind_bwd, s.elt_out, s.elt_in, s.perm, s.dims_after, pv.perm,
-                       s.dims_before, pp);
-    if (r) return r;
-    p.bwd.push_back(pp);
+  {
+    int32_t cur[kMaxDims];
+    memcpy(cur, p.stages[N - 1].perm, sizeof(cur));
+    for (int n = N - 1; n >= 0; --n) {
+      const Stage& s = p.stages[n];
+      int32_t outp[kMaxDims];
+      if (n == 0) memcpy(outp, s.perm, sizeof(outp));
+      else choose(cur, p.stages[n - 1].perm, s.dims_before, outp);
+      PassPlan pp;
+      int r = build_pass(p, n, s.kind_bwd, s.elt_out, s.elt_in, cur, s.dims_after, outp,
+                         s.dims_before, pp);
+      if (r) return r;
+      p.bwd.push_back(pp);
+      memcpy(cur, outp, sizeof(cur));
+    }
   }
   // buffers: user in -> work0 -> work1 -> work0 ... -> user out
   size_t w[2] = {0, 0};

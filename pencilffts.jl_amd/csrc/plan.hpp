@@ -102,6 +102,7 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
                    hipStream_t stream);
 int device_set_timing(Plan& p, bool on);
 int device_stage_times(Plan& p, int n, float* ms);
+int device_pass_names(Plan& p, int dir, std::string* out);
 size_t eltype_bytes(int32_t eltype);
 
 }  // namespace pfft

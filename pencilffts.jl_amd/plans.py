@@ -223,6 +223,12 @@ class PencilFFTPlan:
         L.check(L.lib.pfft_plan_kinds(self._h, f, b))
         return tuple(f), tuple(b)
 
+    def kernels(self, direction=L.FORWARD):
+        """Kernel of each pass of a direction, in launch order."""
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib.pfft_plan_kernels(self._h, direction, buf, 4096))
+        return buf.value.decode().split(";")
+
     def set_timing(self, on=True):
         L.check(L.lib.pfft_plan_set_timing(self._h, int(on)))
 
