@@ -10,6 +10,7 @@ import os
 import shlex
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -46,23 +47,25 @@ def _run(cmd, verbose):
 
 def build(verbose=False):
     os.makedirs(LIB, exist_ok=True)
-    objs = []
+    jobs, objs, eobjs = [], [], []
     for src in PRODUCT_SOURCES:
         obj = os.path.join(LIB, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         dev = DEVICE_FLAGS if src.endswith(".hip") else []
-        _run([HIPCC] + COMMON + dev + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        jobs.append([HIPCC] + COMMON + dev + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # host emulator (tests only): host compilation of the same phase functions
+    for src in EMU_SOURCES:
+        obj = os.path.join(LIB, "emu_" + src.replace(".", "_") + ".o")
+        jobs.append([HIPCC] + COMMON + ["-x", "hip", "--offload-host-only", "-O2",
+                                        "-c", os.path.join(CSRC, src), "-o", obj])
+        eobjs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd in jobs]:
+            f.result()
     out = os.path.join(LIB, "libpencilffts_hip.so")
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-o", out + ".tmp"] + objs, verbose)
     os.replace(out + ".tmp", out)
-    # host emulator (tests only): host compilation of the same phase functions
-    eobjs = []
-    for src in EMU_SOURCES:
-        obj = os.path.join(LIB, "emu_" + src.replace(".", "_") + ".o")
-        _run([HIPCC] + COMMON + ["-x", "hip", "--offload-host-only", "-O2",
-                                 "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        eobjs.append(obj)
     eout = os.path.join(LIB, "libpencilffts_emu.so")
     _run([HIPCC, "-shared", "-fPIC", "-o", eout + ".tmp"] + eobjs, verbose)
     os.replace(eout + ".tmp", eout)

@@ -187,199 +187,229 @@ __device__ __forceinline__ int tile_idx(int k, int t) {
   else return k * T + (t ^ (k & (T - 1)));
 }
 
-template <typename R, int T>
-__device__ __forceinline__ void tile_store(C<R>* tile, int rows, const PassArgs& a, int64_t base,
-                                           int tid, int nthr) {
+// Cooperative tile copies.  The trip count is a compile-time constant so the
+// loops unroll: every global load of a thread is issued before the first LDS
+// write (many 16-byte loads in flight per thread), and likewise for stores.
+template <typename R, int T, int ROWS, int NTHR>
+__device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, int64_t base,
+                                           int tid) {
   C<R>* out = (C<R>*)a.out;
-  if constexpr (sizeof(R) == 4) {
-    constexpr int HP = T / 2;  // float4 pairs per row
-    const int total = rows * HP;
-    for (int e = tid; e < total; e += nthr) {
-      const int k = e / HP;
-      const int t = 2 * (e - k * HP);
-      const float4 val = *(const float4*)&tile[tile_idx<R, T>(k, t)];
-      *(float4*)&out[base + (int64_t)k * a.sk_out + t] = val;
+  constexpr int EPV = sizeof(R) == 4 ? 2 : 1;      // complex per 16-byte vector
+  constexpr int VPR = T / EPV;                     // vectors per row
+  constexpr int TOTAL = ROWS * VPR;
+  constexpr int IT = (TOTAL + NTHR - 1) / NTHR;
+  float4 val[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NTHR;
+    if (TOTAL % NTHR == 0 || e < TOTAL) {
+      const int k = e / VPR;
+      const int t = EPV * (e - k * VPR);
+      val[i] = *(const float4*)&tile[tile_idx<R, T>(k, t)];
     }
-  } else {
-    const int total = rows * T;
-    for (int e = tid; e < total; e += nthr) {
-      const int k = e / T;
-      const int t = e - k * T;
-      out[base + (int64_t)k * a.sk_out + t] = tile[tile_idx<R, T>(k, t)];
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NTHR;
+    if (TOTAL % NTHR == 0 || e < TOTAL) {
+      const int k = e / VPR;
+      const int t = EPV * (e - k * VPR);
+      *(float4*)&out[base + (int64_t)k * a.sk_out + t] = val[i];
     }
   }
 }
 
-template <typename R, int T>
-__device__ __forceinline__ void tile_load(C<R>* tile, int rows, const PassArgs& a, int64_t base,
-                                          int tid, int nthr) {
+template <typename R, int T, int ROWS, int NTHR>
+__device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t base, int tid) {
   const C<R>* in = (const C<R>*)a.in;
-  if constexpr (sizeof(R) == 4) {
-    constexpr int HP = T / 2;
-    const int total = rows * HP;
-    for (int e = tid; e < total; e += nthr) {
-      const int k = e / HP;
-      const int t = 2 * (e - k * HP);
-      *(float4*)&tile[tile_idx<R, T>(k, t)] =
-          *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
+  constexpr int EPV = sizeof(R) == 4 ? 2 : 1;
+  constexpr int VPR = T / EPV;
+  constexpr int TOTAL = ROWS * VPR;
+  constexpr int IT = (TOTAL + NTHR - 1) / NTHR;
+  float4 val[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NTHR;
+    if (TOTAL % NTHR == 0 || e < TOTAL) {
+      const int k = e / VPR;
+      const int t = EPV * (e - k * VPR);
+      val[i] = *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
     }
-  } else {
-    const int total = rows * T;
-    for (int e = tid; e < total; e += nthr) {
-      const int k = e / T;
-      const int t = e - k * T;
-      tile[tile_idx<R, T>(k, t)] = in[base + (int64_t)k * a.sk_in + t];
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * NTHR;
+    if (TOTAL % NTHR == 0 || e < TOTAL) {
+      const int k = e / VPR;
+      const int t = EPV * (e - k * VPR);
+      *(float4*)&tile[tile_idx<R, T>(k, t)] = val[i];
     }
   }
 }
 
-template <typename R, int M, int T, int KIND, int LD, int ST>
+// Lines of a tile: T; lines in flight per pass over the workgroup: TQ =
+// T / LPW (one per L lanes), so each line slot handles LPW lines in turn.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
 constexpr size_t lds_bytes() {
-  constexpr size_t lines = (size_t)T * line_pitch<M>() * sizeof(C<R>);
+  constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
   constexpr int rows = KIND == K_C2C ? M : M + 1;
   constexpr size_t tile = (LD == IO_TILE || ST == IO_TILE) ? (size_t)rows * T * sizeof(C<R>) : 0;
   return lines > tile ? lines : tile;
 }
 
-template <typename R, int M, int T, int KIND, int LD, int ST>
-__global__ __launch_bounds__(T * Cfg<M>::L) void k_fast(PassArgs a) {
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+__global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M>::L, P = Cfg<M>::P;
-  constexpr int NTHR = T * L;
+  constexpr int TQ = T / LPW;
+  constexpr int NTHR = TQ * L;
+  static_assert(TQ * LPW == T, "LPW must divide T");
   extern __shared__ __align__(16) char smem[];
   C<R>* lds = (C<R>*)smem;
   const int tid = threadIdx.x;
-  const int t = tid / L;
-  const int l = tid - t * L;
-  C<R>* line = lds + t * line_pitch<M>();
+  const int g = tid / L;         // line slot
+  const int l = tid - g * L;     // lane within the line
+  C<R>* line = lds + g * line_pitch<M>();
   C<R>* tile = lds;
   const C<R>* __restrict__ tw = (const C<R>*)a.tw;
   const C<R>* __restrict__ aux = (const C<R>*)a.aux;
   const bool conj_io = KIND == K_C2C ? a.bwd != 0 : KIND == K_C2R;
   const R sc = a.do_scale ? (R)a.scale : R(1);
+  constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
 
   for (int64_t tl = blockIdx.x; tl < a.ntiles; tl += gridDim.x) {
     TileCtx c;
     tile_init(a, tl, c);
-    const bool active = t < c.Tact;
-    C<R> v[P];
+    C<R> v[LPW][NX];
 
-    // ---- load ----
-    if constexpr (KIND == K_C2R) {
-      if constexpr (LD == IO_TILE) {
-        tile_load<R, T>(tile, M + 1, a, c.in_base, tid, NTHR);
-        __syncthreads();
-      } else {
-        const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
-        if (active) {
-          for (int k = l; k <= M; k += L) line[pad8(k)] = in[k];
-        }
-        wave_sync();
-      }
-      // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const int k = l + L * p;
-        C<R> A, B;
-        if constexpr (LD == IO_TILE) {
-          A = tile[tile_idx<R, T>(k, t)];
-          B = tile[tile_idx<R, T>(M - k, t)];
-        } else {
-          A = line[pad8(k)];
-          B = line[pad8(M - k)];
-        }
-        B = conj(B);
-        if (k == 0) { A.im = R(0); B.im = R(0); }
-        const C<R> E = A + B;
-        const C<R> O = (A - B) * conj(aux[k]);
-        v[p] = conj(E + mul_pi(O));   // conj: backward FFT as conj(FFT(conj))
-      }
-      if constexpr (LD == IO_TILE) __syncthreads();
-      else wave_sync();
-    } else if constexpr (LD == IO_TILE) {
-      tile_load<R, T>(tile, M, a, c.in_base, tid, NTHR);
+    // ---------------- load (+ c2r merge) ----------------
+    if constexpr (LD == IO_TILE) {
+      tile_load<R, T, KIND == K_C2R ? M + 1 : M, NTHR>(tile, a, c.in_base, tid);
       __syncthreads();
+    }
 #pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const C<R> x = tile[tile_idx<R, T>(l + L * p, t)];
-        v[p] = conj_io ? conj(x) : x;
-      }
-      __syncthreads();
-    } else {
-      // contiguous line: complex elements, or real pairs (x[2m], x[2m+1]) for r2c
-      const C<R>* in = KIND == K_R2C
-                           ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
-                           : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
-      if (active) {
+    for (int q = 0; q < LPW; ++q) {
+      const int t = q * TQ + g;
+      const bool active = t < c.Tact;
+      if constexpr (KIND == K_C2R) {
+        if constexpr (LD == IO_LINE) {
+          const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+          if (active)
+            for (int k = l; k <= M; k += L) line[pad8(k)] = in[k];
+          wave_sync();
+        }
+        // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          const C<R> x = in[l + L * p];
-          v[p] = conj_io ? conj(x) : x;
+          const int k = l + L * p;
+          C<R> A, B;
+          if constexpr (LD == IO_TILE) {
+            A = tile[tile_idx<R, T>(k, t)];
+            B = tile[tile_idx<R, T>(M - k, t)];
+          } else {
+            A = line[pad8(k)];
+            B = line[pad8(M - k)];
+          }
+          B = conj(B);
+          if (k == 0) { A.im = R(0); B.im = R(0); }
+          const C<R> E = A + B;
+          const C<R> O = (A - B) * conj(aux[k]);
+          v[q][p] = conj(E + mul_pi(O));   // backward FFT as conj(FFT(conj))
+        }
+        if constexpr (LD == IO_LINE) wave_sync();
+      } else if constexpr (LD == IO_TILE) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const C<R> x = tile[tile_idx<R, T>(l + L * p, t)];
+          v[q][p] = conj_io ? conj(x) : x;
+        }
+      } else {
+        // contiguous line: complex elements, or real pairs (x[2m], x[2m+1]) for r2c
+        const C<R>* in = KIND == K_R2C
+                             ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
+                             : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+        if (active) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const C<R> x = in[l + L * p];
+            v[q][p] = conj_io ? conj(x) : x;
+          }
         }
       }
     }
+    if constexpr (LD == IO_TILE) __syncthreads();   // line slots alias the tile
 
-    // ---- FFT ----
-    Cfg<M>::template fft<R>(v, line, l, tw);
-
-    // ---- post + store ----
-    if constexpr (KIND == K_R2C) {
-      // split: X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
-      wave_sync();
+    // ---------------- FFT (+ r2c split, line stores) ----------------
 #pragma unroll
-      for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[p];
-      wave_sync();
-      C<R> X[P + 1];
+    for (int q = 0; q < LPW; ++q) {
+      const int t = q * TQ + g;
+      const bool active = t < c.Tact;
+      if (q > 0) wave_sync();
+      Cfg<M>::template fft<R>(v[q], line, l, tw);
+      if constexpr (KIND == K_R2C) {
+        // X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
+        wave_sync();
 #pragma unroll
-      for (int p = 0; p <= P; ++p) {
-        const int k = p < P ? l + L * p : M;
-        const C<R> Z = line[pad8(k & (M - 1))];
-        const C<R> Zc = conj(line[pad8((M - k) & (M - 1))]);
-        const C<R> E = scl(Z + Zc, R(0.5));
-        const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
-        X[p] = E + aux[k] * O;
-      }
-      if constexpr (ST == IO_TILE) {
-        __syncthreads();
+        for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[q][p];
+        wave_sync();
+        C<R> X[P + 1];
 #pragma unroll
-        for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = X[p];
-        if (l == 0) tile[tile_idx<R, T>(M, t)] = X[P];
-        __syncthreads();
-        tile_store<R, T>(tile, M + 1, a, c.out_base, tid, NTHR);
-      } else {
-        C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+        for (int p = 0; p <= P; ++p) {
+          const int k = p < P ? l + L * p : M;
+          const C<R> Z = line[pad8(k & (M - 1))];
+          const C<R> Zc = conj(line[pad8((M - k) & (M - 1))]);
+          const C<R> E = scl(Z + Zc, R(0.5));
+          const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
+          X[p] = E + aux[k] * O;
+        }
+#pragma unroll
+        for (int p = 0; p <= P; ++p) v[q][p] = X[p];
+        if constexpr (ST == IO_LINE) {
+          C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+          if (active) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) out[l + L * p] = X[p];
+            if (l == 0) out[M] = X[P];
+          }
+        }
+      } else if constexpr (KIND == K_C2R) {
+        // z_m -> real pairs x[2m] = Re z_m, x[2m+1] = Im z_m
+        C<R>* out = (C<R>*)((R*)a.out + c.out_base + (int64_t)t * a.lsout[0]);
         if (active) {
 #pragma unroll
-          for (int p = 0; p < P; ++p) out[l + L * p] = X[p];
-          if (l == 0) out[M] = X[P];
+          for (int p = 0; p < P; ++p) out[l + L * p] = scl(conj(v[q][p]), sc);
         }
-      }
-    } else if constexpr (KIND == K_C2R) {
-      // z_m = conj(FFT) -> real pairs x[2m] = Re z_m, x[2m+1] = Im z_m
-      C<R>* out = (C<R>*)((R*)a.out + c.out_base + (int64_t)t * a.lsout[0]);
-      if (active) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) out[l + L * p] = scl(conj(v[p]), sc);
-      }
-    } else {
-      if constexpr (ST == IO_TILE) {
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-          const C<R> y = conj_io ? conj(v[p]) : v[p];
-          tile[tile_idx<R, T>(l + L * p, t)] = scl(y, sc);
-        }
-        __syncthreads();
-        tile_store<R, T>(tile, M, a, c.out_base, tid, NTHR);
-      } else {
+      } else if constexpr (ST == IO_LINE) {
         C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
         if (active) {
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const C<R> y = conj_io ? conj(v[p]) : v[p];
+            const C<R> y = conj_io ? conj(v[q][p]) : v[q][p];
             out[l + L * p] = scl(y, sc);
           }
         }
       }
+    }
+
+    // ---------------- transposed store ----------------
+    if constexpr (ST == IO_TILE) {
+      __syncthreads();   // every line slot is read; the tile may overwrite them
+#pragma unroll
+      for (int q = 0; q < LPW; ++q) {
+        const int t = q * TQ + g;
+        if constexpr (KIND == K_R2C) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = v[q][p];
+          if (l == 0) tile[tile_idx<R, T>(M, t)] = v[q][P];
+        } else {
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const C<R> y = conj_io ? conj(v[q][p]) : v[q][p];
+            tile[tile_idx<R, T>(l + L * p, t)] = scl(y, sc);
+          }
+        }
+      }
+      __syncthreads();
+      tile_store<R, T, KIND == K_R2C ? M + 1 : M, NTHR>(tile, a, c.out_base, tid);
     }
     __syncthreads();
   }
@@ -387,19 +417,19 @@ __global__ __launch_bounds__(T * Cfg<M>::L) void k_fast(PassArgs a) {
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, threads;
+  int real_bytes, M, kind, ld, st, T, lpw, threads;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
-template <typename R, int M, int T, int KIND, int LD, int ST>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, KIND, LD, ST>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST>), g, b, lds, s, a);
 }
 
-template <typename R, int M, int T, int KIND, int LD, int ST>
-static Entry make() {
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
   e.M = M;
@@ -407,29 +437,49 @@ static Entry make() {
   e.ld = LD;
   e.st = ST;
   e.T = T;
-  e.threads = T * Cfg<M>::L;
-  e.lds = lds_bytes<R, M, T, KIND, LD, ST>();
-  e.fn = (const void*)k_fast<R, M, T, KIND, LD, ST>;
-  e.launch = launch_one<R, M, T, KIND, LD, ST>;
-  return e;
+  e.lpw = LPW;
+  e.threads = T / LPW * Cfg<M>::L;
+  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST>();
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST>;
+  v.push_back(e);
 }
 
-// lines per workgroup: 16 (f32) / 8 (f64) for transposed tiles (128-byte
-// runs); 256 threads per workgroup for contiguous passes
-template <typename R, int M>
-constexpr int t_tile() { return (sizeof(R) == 4 && M < 1024) ? 16 : 8; }
-template <int M>
-constexpr int t_line() { return 256 / Cfg<M>::L; }
-
+// Candidates per (precision, length, kind, load, store).  The first of each
+// group is the default when autotuning is off; the autotuner times them all.
 template <typename R, int M>
 static void add_size(std::vector<Entry>& v) {
-  v.push_back(make<R, M, t_line<M>(), K_C2C, IO_LINE, IO_LINE>());
-  v.push_back(make<R, M, t_tile<R, M>(), K_C2C, IO_LINE, IO_TILE>());
-  v.push_back(make<R, M, t_tile<R, M>(), K_C2C, IO_TILE, IO_LINE>());
-  v.push_back(make<R, M, t_line<M>(), K_R2C, IO_LINE, IO_LINE>());
-  v.push_back(make<R, M, t_tile<R, M>(), K_R2C, IO_LINE, IO_TILE>());
-  v.push_back(make<R, M, t_line<M>(), K_C2R, IO_LINE, IO_LINE>());
-  v.push_back(make<R, M, t_tile<R, M>(), K_C2R, IO_TILE, IO_LINE>());
+  constexpr int TL = 256 / Cfg<M>::L;       // contiguous passes: 256 threads
+  constexpr int TT = sizeof(R) == 4 ? 16 : 8;  // transposed runs of 128 bytes
+  add<R, M, TL, 1, K_C2C, IO_LINE, IO_LINE>(v);
+  add<R, M, TL * 2, 2, K_C2C, IO_LINE, IO_LINE>(v);
+  add<R, M, TL, 1, K_R2C, IO_LINE, IO_LINE>(v);
+  add<R, M, TL, 1, K_C2R, IO_LINE, IO_LINE>(v);
+  if constexpr (M < 1024) {
+    add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, TT, 1, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, TT, 4, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, TT, 2, K_C2C, IO_TILE, IO_LINE>(v);
+    add<R, M, TT, 1, K_C2C, IO_TILE, IO_LINE>(v);
+    add<R, M, TT, 4, K_C2C, IO_TILE, IO_LINE>(v);
+    add<R, M, TT, 2, K_R2C, IO_LINE, IO_TILE>(v);
+    add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE>(v);
+    add<R, M, TT, 2, K_C2R, IO_TILE, IO_LINE>(v);
+    add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE>(v);
+    if constexpr (sizeof(R) == 4) {
+      add<R, M, 32, 4, K_C2C, IO_LINE, IO_TILE>(v);
+      add<R, M, 32, 4, K_C2C, IO_TILE, IO_LINE>(v);
+      add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE>(v);
+      add<R, M, 32, 2, K_C2R, IO_TILE, IO_LINE>(v);
+    }
+  } else {
+    add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, 8, 1, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, 8, 2, K_C2C, IO_TILE, IO_LINE>(v);
+    add<R, M, 8, 1, K_C2C, IO_TILE, IO_LINE>(v);
+    add<R, M, 8, 2, K_R2C, IO_LINE, IO_TILE>(v);
+    add<R, M, 8, 2, K_C2R, IO_TILE, IO_LINE>(v);
+  }
 }
 
 static const std::vector<Entry>& registry() {
@@ -459,44 +509,54 @@ static bool all_even(const int64_t* s, int from, int D) {
   return true;
 }
 
-void fast_select(const Plan& p, PassPlan& pp) {
-  if (p.flags & PFFT_FLAG_GENERIC_ONLY) return;
-  PassArgs& a = pp.args;
+// Specialised kernels that can run a pass (first = default choice).
+std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp) {
+  std::vector<int> out;
+  if (p.flags & PFFT_FLAG_GENERIC_ONLY) return out;
+  const PassArgs& a = pp.args;
   int kind;
   if (a.kind == L_C2C) kind = K_C2C;
   else if (a.kind == L_R2C_HALF) kind = K_R2C;
   else if (a.kind == L_C2R_HALF) kind = K_C2R;
-  else return;
+  else return out;
   const int rb = p.real_type == PFFT_F32 ? 4 : 8;
   const int M = (int)a.M;
   int ld, st;
   if (a.sk_in == 1) ld = IO_LINE;
   else if (a.lsin[0] == 1 && kind != K_R2C) ld = IO_TILE;
-  else return;
+  else return out;
   if (a.sk_out == 1) st = IO_LINE;
   else if (a.lsout[0] == 1 && kind != K_C2R) st = IO_TILE;
-  else return;
-  if (ld == IO_TILE && st == IO_TILE) return;
-  const std::vector<Entry>& reg = registry();
-  const Entry* e = nullptr;
-  for (const Entry& x : reg)
-    if (x.real_bytes == rb && x.M == M && x.kind == kind && x.ld == ld && x.st == st) e = &x;
-  if (!e) return;
-  if (e->lds > 160 * 1024) return;
-  // transposed tiles: full tiles, 16-byte aligned runs
-  if (ld == IO_TILE || st == IO_TILE) {
-    if (a.ldim[0] % e->T) return;
-    const int64_t sk = ld == IO_TILE ? a.sk_in : a.sk_out;
-    const int64_t* ls = ld == IO_TILE ? a.lsin : a.lsout;
-    if (rb == 4 && ((sk & 1) || !all_even(ls, 1, a.D))) return;
-  }
+  else return out;
+  if (ld == IO_TILE && st == IO_TILE) return out;
   // real pairs of r2c input / c2r output need even line offsets
-  if (kind == K_R2C && !all_even(a.lsin, 0, a.D)) return;
-  if (kind == K_C2R && !all_even(a.lsout, 0, a.D)) return;
-  // contiguous passes: put a line dim that the tile size divides first
-  if (ld == IO_LINE && st == IO_LINE && a.ldim[0] % e->T) {
+  if (kind == K_R2C && !all_even(a.lsin, 0, a.D)) return out;
+  if (kind == K_C2R && !all_even(a.lsout, 0, a.D)) return out;
+  const std::vector<Entry>& reg = registry();
+  for (size_t i = 0; i < reg.size(); ++i) {
+    const Entry& x = reg[i];
+    if (x.real_bytes != rb || x.M != M || x.kind != kind || x.ld != ld || x.st != st) continue;
+    if (x.lds > 160 * 1024) continue;
+    if (ld == IO_TILE || st == IO_TILE) {
+      // transposed tiles: full tiles, 16-byte aligned runs
+      if (a.ldim[0] % x.T) continue;
+      const int64_t sk = ld == IO_TILE ? a.sk_in : a.sk_out;
+      const int64_t* ls = ld == IO_TILE ? a.lsin : a.lsout;
+      if (rb == 4 && ((sk & 1) || !all_even(ls, 1, a.D))) continue;
+    }
+    out.push_back((int)i + 1);
+  }
+  return out;
+}
+
+void fast_apply(PassPlan& pp, int variant) {
+  const Entry& e = registry()[variant - 1];
+  PassArgs& a = pp.args;
+  // contiguous passes: keep the input-adjacent line dim first (its partial
+  // last tile costs little) unless it is shorter than a tile
+  if (e.ld == IO_LINE && e.st == IO_LINE && a.ldim[0] < e.T) {
     for (int d = 1; d < a.D; ++d) {
-      if (a.ldim[d] % e->T == 0) {
+      if (a.ldim[d] % e.T == 0) {
         std::swap(a.ldim[0], a.ldim[d]);
         std::swap(a.lsin[0], a.lsin[d]);
         std::swap(a.lsout[0], a.lsout[d]);
@@ -504,14 +564,14 @@ void fast_select(const Plan& p, PassPlan& pp) {
       }
     }
   }
-  a.T = e->T;
-  int64_t nt = (a.ldim[0] + e->T - 1) / e->T;
+  a.T = e.T;
+  int64_t nt = (a.ldim[0] + e.T - 1) / e.T;
   for (int d = 1; d < a.D; ++d) nt *= a.ldim[d];
   a.ntiles = nt;
   pp.grid = std::min<int64_t>(nt, 1 << 20);
-  pp.threads = e->threads;
-  pp.lds_bytes = (int64_t)e->lds;
-  pp.variant = 1 + (int)(e - reg.data());
+  pp.threads = e.threads;
+  pp.lds_bytes = (int64_t)e.lds;
+  pp.variant = variant;
 }
 
 int fast_prepare(PassPlan& pp) {
@@ -543,7 +603,8 @@ const char* fast_name(int variant) {
   static const char* kinds[3] = {"c2c", "r2c", "c2r"};
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
-      (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ">";
+      (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
+      std::to_string(e.lpw) + ">";
   return s.c_str();
 }
 

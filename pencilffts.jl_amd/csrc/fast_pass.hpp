@@ -5,9 +5,10 @@
 
 namespace pfft {
 
-// Picks a specialised kernel for the pass when one exists (sets pp.variant,
-// threads, grid, lds_bytes); leaves the generic pass otherwise.
-void fast_select(const Plan& p, PassPlan& pp);
+// Specialised kernels able to run a pass, as variant ids (first = default).
+std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp);
+// Switches a pass to a specialised variant (tile shape, grid, LDS).
+void fast_apply(PassPlan& pp, int variant);
 int fast_prepare(PassPlan& pp);
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream);
 const char* fast_name(int variant);  // kernel name of a variant ("generic" for 0)

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 
 #include "../../include/pfft.h"
@@ -56,6 +59,119 @@ static int set_lds_limit(const void* fn, int64_t bytes) {
   return PFFT_OK;
 }
 
+// ---- autotuner ------------------------------------------------------------------
+// A pass can usually run on several specialised variants (tile shape, lines
+// per wave).  The first use of a pass shape times every candidate on scratch
+// buffers of the pass's size and keeps the fastest; the choice is cached per
+// process by the pass signature.  Variants differ only in how work is spread,
+// not in arithmetic, so results do not depend on the choice.
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_tune_cache;
+
+static std::string pass_key(const Plan& p, const PassPlan& pp) {
+  const PassArgs& a = pp.args;
+  std::ostringstream o;
+  o << p.real_type << ':' << a.kind << ':' << a.M << ':' << a.n << ':' << a.L_in << ':' << a.L_out
+    << ':' << a.sk_in << ':' << a.sk_out << ':' << a.in_cplx << a.out_cplx << ':' << a.D;
+  for (int d = 0; d < a.D; ++d) o << ':' << a.ldim[d] << ',' << a.lsin[d] << ',' << a.lsout[d];
+  return o.str();
+}
+
+static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, void* out,
+                        hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
+  PassPlan pp = base;
+  fast_apply(pp, variant);
+  int r = fast_prepare(pp);
+  if (r) return r;
+  PassArgs a = pp.args;
+  a.in = in;
+  a.out = out;
+  a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
+  a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+  r = fast_launch(pp, a, st);  // warm-up
+  if (r) return r;
+  HIP_TRY(hipEventRecord(e0, st));
+  const int reps = 3;
+  for (int i = 0; i < reps; ++i) {
+    r = fast_launch(pp, a, st);
+    if (r) return r;
+  }
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipGetLastError());
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  return PFFT_OK;
+}
+
+static int autotune(Plan& p) {
+  struct Job { PassPlan* pp; std::vector<int> cands; std::string key; };
+  std::vector<Job> jobs;
+  size_t in_max = 0, out_max = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto* list : {&p.fwd, &p.bwd}) {
+      for (PassPlan& pp : *list) {
+        std::vector<int> c = fast_candidates(p, pp);
+        if (c.empty()) continue;
+        const std::string key = pass_key(p, pp);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+          fast_apply(pp, it->second);
+          continue;
+        }
+        if (c.size() == 1 || (p.flags & PFFT_FLAG_NO_AUTOTUNE)) {
+          fast_apply(pp, c[0]);
+          continue;
+        }
+        jobs.push_back({&pp, c, key});
+        in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
+        out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
+      }
+    }
+  }
+  if (jobs.empty()) return PFFT_OK;
+  void *in = nullptr, *out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PFFT_OK;
+  auto cleanup = [&]() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  if (hipMalloc(&in, in_max) != hipSuccess || hipMalloc(&out, out_max) != hipSuccess ||
+      hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    // no room to tune: fall back to the default variants
+    for (Job& j : jobs) fast_apply(*j.pp, j.cands[0]);
+    (void)hipGetLastError();
+    return PFFT_OK;
+  }
+  (void)hipMemsetAsync(in, 0, in_max, st);
+  for (Job& j : jobs) {
+    int best = j.cands[0];
+    float best_ms = 1e30f;
+    for (int v : j.cands) {
+      float ms = 0;
+      rc = time_variant(p, *j.pp, v, in, out, st, e0, e1, &ms);
+      if (rc) break;
+      if (ms < best_ms) { best_ms = ms; best = v; }
+    }
+    if (rc) break;
+    fast_apply(*j.pp, best);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[j.key] = best;
+  }
+  cleanup();
+  return rc;
+}
+
 int device_prepare(Plan& p) {
   if (p.dev) return PFFT_OK;
   DeviceState* d = new DeviceState();
@@ -76,10 +192,13 @@ int device_prepare(Plan& p) {
       return fail(PFFT_ERR_DEVICE, std::string("hipMemcpy(tables): ") + hipGetErrorString(e));
     }
   }
+  int r = autotune(p);
+  if (r) {
+    delete d;
+    return r;
+  }
   for (auto* list : {&p.fwd, &p.bwd}) {
     for (PassPlan& pp : *list) {
-      fast_select(p, pp);
-      int r = PFFT_OK;
       if (pp.variant == 0) {
         r = p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
                                     : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);

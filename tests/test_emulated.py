@@ -1,0 +1,92 @@
+"""CPU parity of the generic pass with the oracle, through the host emulator.
+
+libpencilffts_emu.so runs the very phase functions of the generic GPU kernel
+(generic_pass.hpp) thread by thread on the CPU, with the same plan, pass
+descriptors and precision-rounded twiddle tables as the product library.
+These tests cover the kernel arithmetic and index maps on the reference's
+test matrix without a GPU; the -m gpu tests repeat them on the device."""
+import numpy as np
+import pytest
+
+from oracle import pipeline as OP
+from oracle import plan as OM
+from tests.cases import (CODE_NAMES, DATA_DIMS, NP_DTYPES, meta_for, random_logical, tol_for,
+                         transform_list)
+
+
+def emu_case(pf, emu, dims, trs, real_type, permute=True, extra=(), seed=0):
+    T = pf.Transforms
+    real_name = CODE_NAMES[real_type]
+    meta, full = meta_for(T, dims, trs, real_name, permute, extra)
+    kinds = [t.code for t in full]
+    rng = np.random.default_rng(seed)
+    is_c2r = full[-1].code in (pf._lib.BRFFT_EVEN, pf._lib.BRFFT_ODD)
+    dt_in = NP_DTYPES[meta.eltype_in]
+    dt_out = NP_DTYPES[meta.eltype_out]
+    if is_c2r:
+        y = random_logical(rng, tuple(meta.size_out) + tuple(extra), meta.eltype_out)
+        ym = OP.memory_image(OP.to_memory(y, meta.perm_out)).astype(dt_out)
+        xin = emu.run(dims, kinds, real_type, 2, ym, permute, extra).view(dt_in)
+    else:
+        x = random_logical(rng, tuple(dims) + tuple(extra), meta.eltype_in)
+        xin = OP.memory_image(OP.to_memory(x, meta.perm_in)).astype(dt_in)
+    xw = xin.astype(np.complex128 if np.iscomplexobj(xin) else np.float64)
+    out = emu.run(dims, kinds, real_type, 0, xin, permute, extra).view(dt_out)
+    xm = OP.from_image(xw, meta.mem_dims("in"))
+    ref = OP.memory_image(OP.forward(meta, xm))
+    err_f = OP.rel_l2(out, ref)
+    back = emu.run(dims, kinds, real_type, 2, out, permute, extra).view(dt_in)
+    err_b = OP.rel_l2(back, xw)
+    return err_f, err_b
+
+
+@pytest.mark.parametrize("extra", [(), (3,)], ids=["noextra", "extra3"])
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("idx", range(16))
+def test_transform_list(pf, emu, idx, real_type, extra):
+    """test/transforms.jl:312-337 at DATA_DIMS (16, 12, 6)."""
+    T = pf.Transforms
+    trs = transform_list(T, DATA_DIMS)[idx]
+    err_f, err_b = emu_case(pf, emu, DATA_DIMS, trs, real_type, extra=extra, seed=idx)
+    tol = tol_for(CODE_NAMES[real_type], DATA_DIMS)
+    assert err_f <= tol, err_f
+    assert err_b <= tol, err_b
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims,permute", [((11, 15), True), ((11, 7, 13), True),
+                                          ((9, 7, 5, 12), True), ((9, 7, 5, 12), False),
+                                          ((42, 24, 16), True), ((41, 23, 15), True),
+                                          ((13,), True)])
+def test_dimensionality_and_sizes(pf, emu, dims, permute, real_type):
+    """transforms.jl:414-462 and the rfft.jl sizes (42,24,16) / (41,23,15)."""
+    T = pf.Transforms
+    for trs in [(T.RFFT(),), (T.FFT(),)]:
+        err_f, err_b = emu_case(pf, emu, dims, trs, real_type, permute=permute)
+        tol = tol_for(CODE_NAMES[real_type], dims)
+        assert err_f <= tol and err_b <= tol, (trs, err_f, err_b)
+
+
+def test_mixed_config3_small(pf, emu):
+    """(RFFT, FFT, R2R(REDFT10)) -- the mixed workload's transform tuple -- at a
+    small size, both precisions."""
+    T = pf.Transforms
+    trs = (T.RFFT(), T.FFT(), T.R2R(T.REDFT10))
+    for rt in (0, 1):
+        err_f, err_b = emu_case(pf, emu, (24, 12, 10), trs, rt)
+        tol = tol_for(CODE_NAMES[rt], (24, 12, 10))
+        assert err_f <= tol and err_b <= tol
+
+
+def test_index_map_bit_exact(pf, emu):
+    T = pf.Transforms
+    for dims, permute, extra in [((16, 12, 6), True, ()), ((5, 7, 3), True, (2,)),
+                                 ((9, 7, 5, 12), True, ()), ((9, 7, 5, 12), False, ())]:
+        n = int(np.prod(dims)) * int(np.prod(extra))
+        vals = np.arange(n, dtype=np.float64)
+        meta = OM.PlanMeta(dims, ("NoTransform",), "Float64", permute, extra)
+        out = emu.run(dims, [T.NoTransform().code] * len(dims), 1, 0, vals, permute,
+                      extra).view(np.float64)
+        want = OP.memory_image(OP.to_memory(OP.from_image(vals, tuple(dims) + tuple(extra)),
+                                            meta.perm_out))
+        assert np.array_equal(out, want)
