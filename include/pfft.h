@@ -63,7 +63,16 @@ enum { PFFT_F32 = 0, PFFT_F64 = 1, PFFT_C32 = 2, PFFT_C64 = 3, PFFT_NOTHING = -1
 enum {
   PFFT_FORWARD = 0,          /* mul!(dst, p, src)          operations.jl:24-31 */
   PFFT_BACKWARD = 1,         /* PencilFFTs.bmul!(...)      operations.jl:34-41 */
-  PFFT_BACKWARD_SCALED = 2   /* ldiv!(dst, p, src)         operations.jl:44-52 */
+  PFFT_BACKWARD_SCALED = 2,  /* ldiv!(dst, p, src)         operations.jl:44-52 */
+  /* Adjoints (transposes for the real inner product Re sum conj(a) b) of the
+   * two unnormalised maps, for reverse-mode differentiation.  No counterpart
+   * in the reference; they back the ChainRulesCore rrule of INTEGRATION.md.
+   * The adjoint of an r2c stage is the c2r of the spectrum divided by the
+   * Hermitian weights w_k (1 at DC and, for even n, Nyquist; 2 elsewhere);
+   * the adjoint of a c2r stage is w_k times the r2c.  Plans with an R2R
+   * stage return PFFT_ERR_UNSUPPORTED. */
+  PFFT_ADJOINT_FORWARD = 3,  /* x_bar = (mul!)^T y_bar: output-shaped -> input-shaped */
+  PFFT_ADJOINT_BACKWARD = 4  /* y_bar = (bmul!)^T x_bar: input-shaped -> output-shaped */
 };
 
 /* ---- plan flags -------------------------------------------------------- */

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libpencilffts_hip.so")
 OK, ERR_ARGUMENT, ERR_UNSUPPORTED, ERR_DEVICE, ERR_ALLOC = 0, 1, 2, 3, 4
 NOTRANSFORM, FFT, BFFT, RFFT, BRFFT_EVEN, BRFFT_ODD, R2R, INPLACE = 0, 1, 2, 3, 4, 5, 16, 0x100
 F32, F64, C32, C64, NOTHING = 0, 1, 2, 3, -1
-FORWARD, BACKWARD, BACKWARD_SCALED = 0, 1, 2
+FORWARD, BACKWARD, BACKWARD_SCALED, ADJOINT_FORWARD, ADJOINT_BACKWARD = 0, 1, 2, 3, 4
 FLAG_GENERIC_ONLY, FLAG_NO_AUTOTUNE, FLAG_MFMA_SPLIT16, FLAG_MFMA_BF16X3 = 1, 2, 4, 8
 
 # every symbol include/pfft.h declares, with (restype, argtypes)

@@ -72,7 +72,7 @@ struct PassArgs {
   int64_t lds_w1_off;  // byte offset of work buffer 1 (== raw tile) in LDS
   double scale;        // output multiplier (1 unless ldiv! is fused)
   int32_t do_scale;
-  int32_t pad_;
+  int32_t adjoint;     // fuse the Hermitian weights of the r2c/c2r adjoints
 };
 
 template <typename R>

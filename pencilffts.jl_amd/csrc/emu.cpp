@@ -72,11 +72,14 @@ __attribute__((visibility("default"))) int pfft_emu_nbytes(const pfft_emu_plan* 
 __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, int32_t dir,
                                                            const void* in, void* out) {
   Plan& p = h->p;
-  if (dir != PFFT_FORWARD && dir != PFFT_BACKWARD && dir != PFFT_BACKWARD_SCALED)
+  if (dir < PFFT_FORWARD || dir > PFFT_ADJOINT_BACKWARD)
     return fail(PFFT_ERR_ARGUMENT, "unknown direction");
+  if ((dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) && !p.adjoint_ok)
+    return fail(PFFT_ERR_UNSUPPORTED, "no adjoint for R2R plans");
+  const bool fwd_shape = dir == PFFT_FORWARD || dir == PFFT_ADJOINT_BACKWARD;
   // the two work buffers as separate exact-size allocations
   std::vector<char> work0(p.work0_bytes), work1(p.work_bytes - p.work0_bytes);
-  const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+  const std::vector<PassPlan>& passes = fwd_shape ? p.fwd : p.bwd;
   void* bufs[3] = {nullptr, work0.data(), work1.data()};
   const int np = (int)passes.size();
   for (int i = 0; i < np; ++i) {
@@ -90,6 +93,7 @@ __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, in
       a.do_scale = 1;
       a.scale = 1.0 / p.scale;
     }
+    a.adjoint = (dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) ? 1 : 0;
     if (p.real_type == PFFT_F32) emu_pass<float>(pp, a);
     else emu_pass<double>(pp, a);
   }

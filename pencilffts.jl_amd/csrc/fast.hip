@@ -80,7 +80,9 @@ __device__ __forceinline__ void dft16(C<R>* v) {
 template <typename R, int RAD>
 __device__ __forceinline__ void dft(C<R>* v) {
   if constexpr (RAD == 2) dft2(v);
+  else if constexpr (RAD == 3) dft3(v);
   else if constexpr (RAD == 4) dft4(v);
+  else if constexpr (RAD == 5) dft5(v);
   else if constexpr (RAD == 8) dft8(v);
   else if constexpr (RAD == 16) dft16(v);
 }
@@ -108,9 +110,16 @@ __device__ __forceinline__ void stage(C<R>* v, C<R>* line, int l, const C<R>* __
   for (int i = 0; i < BPL; ++i) {
     C<R>* w = u + i * RAD;
     if constexpr (NS > 1) {
+      // one table load per butterfly: w^1 = w_M^(jj M/(Ns RAD)); the other
+      // powers by a product tree at most three multiplies deep
       const int jj = (l + L * i) % NS;
+      const C<R> w1 = tw[jj * (M / (NS * RAD))];
+      C<R> pw[RAD];
+      pw[1] = w1;
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) w[r] = w[r] * tw[r * jj * (M / (NS * RAD))];
+      for (int r = 2; r < RAD; ++r) pw[r] = pw[r / 2] * pw[r - r / 2];
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) w[r] = w[r] * pw[r];
     }
     dft<R, RAD>(w);
   }
@@ -176,7 +185,33 @@ template <> struct Cfg<1024> {
   }
 };
 
-enum { K_C2C = 0, K_R2C = 1, K_C2R = 2 };
+// mixed radix: P is a multiple of every radix of the schedule
+#define PFFT_CFG(MM, LL, PP, ...)                                        \
+  template <> struct Cfg<MM> {                                           \
+    static constexpr int L = LL, P = PP;                                 \
+    template <typename R>                                                \
+    __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) { \
+      fft_rec<R, MM, L, P, 1, true, __VA_ARGS__>(v, line, l, tw);        \
+    }                                                                    \
+  };
+PFFT_CFG(32, 4, 8, 8, 4)
+PFFT_CFG(48, 4, 12, 4, 4, 3)
+PFFT_CFG(96, 8, 12, 4, 4, 3, 2)
+PFFT_CFG(160, 8, 20, 4, 4, 2, 5)
+PFFT_CFG(192, 16, 12, 4, 4, 4, 3)
+PFFT_CFG(320, 16, 20, 4, 4, 4, 5)
+PFFT_CFG(384, 32, 12, 4, 4, 4, 3, 2)
+#undef PFFT_CFG
+
+// kinds of the fast pass:
+//   K_C2C  complex FFT (BFFT as conj(FFT(conj)))
+//   K_R2C  r2c of real length 2M via the M-point FFT + split twiddle
+//   K_C2R  c2r to real length 2M via merge twiddle + M-point FFT
+//   K_DCT2C REDFT10 of complex data: Makhoul reorder, M-point FFT, Hermitian
+//           split of the real/imaginary parts and the post-twiddle
+//   K_DCT3C REDFT01 of complex data: pre-twiddle over (k, M-k), backward
+//           M-point FFT, inverse Makhoul reorder
+enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4 };
 enum { IO_LINE = 0, IO_TILE = 1 };
 
 // tile element (row k, line t): XOR swizzle keeps each row's 16-byte pairs
@@ -187,9 +222,10 @@ __device__ __forceinline__ int tile_idx(int k, int t) {
   else return k * T + (t ^ (k & (T - 1)));
 }
 
-// Cooperative tile copies.  The trip count is a compile-time constant so the
-// loops unroll: every global load of a thread is issued before the first LDS
-// write (many 16-byte loads in flight per thread), and likewise for stores.
+// Cooperative tile copies.  Full rounds (every thread busy) are unrolled
+// with compile-time trip counts, so a thread issues all its global loads
+// before the first LDS write; the partial last round is a separate guarded
+// step that keeps nothing live across its branch.
 template <typename R, int T, int ROWS, int NTHR>
 __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, int64_t base,
                                            int tid) {
@@ -197,24 +233,22 @@ __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, 
   constexpr int EPV = sizeof(R) == 4 ? 2 : 1;      // complex per 16-byte vector
   constexpr int VPR = T / EPV;                     // vectors per row
   constexpr int TOTAL = ROWS * VPR;
-  constexpr int IT = (TOTAL + NTHR - 1) / NTHR;
-  float4 val[IT];
+  constexpr int FULL = TOTAL / NTHR;
 #pragma unroll
-  for (int i = 0; i < IT; ++i) {
+  for (int i = 0; i < FULL; ++i) {
     const int e = tid + i * NTHR;
-    if (TOTAL % NTHR == 0 || e < TOTAL) {
-      const int k = e / VPR;
-      const int t = EPV * (e - k * VPR);
-      val[i] = *(const float4*)&tile[tile_idx<R, T>(k, t)];
-    }
+    const int k = e / VPR;
+    const int t = EPV * (e - k * VPR);
+    *(float4*)&out[base + (int64_t)k * a.sk_out + t] =
+        *(const float4*)&tile[tile_idx<R, T>(k, t)];
   }
-#pragma unroll
-  for (int i = 0; i < IT; ++i) {
-    const int e = tid + i * NTHR;
-    if (TOTAL % NTHR == 0 || e < TOTAL) {
+  if constexpr (TOTAL % NTHR != 0) {
+    const int e = tid + FULL * NTHR;
+    if (e < TOTAL) {
       const int k = e / VPR;
       const int t = EPV * (e - k * VPR);
-      *(float4*)&out[base + (int64_t)k * a.sk_out + t] = val[i];
+      *(float4*)&out[base + (int64_t)k * a.sk_out + t] =
+          *(const float4*)&tile[tile_idx<R, T>(k, t)];
     }
   }
 }
@@ -225,24 +259,23 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
   constexpr int EPV = sizeof(R) == 4 ? 2 : 1;
   constexpr int VPR = T / EPV;
   constexpr int TOTAL = ROWS * VPR;
-  constexpr int IT = (TOTAL + NTHR - 1) / NTHR;
-  float4 val[IT];
+  constexpr int FULL = TOTAL / NTHR;
+  // global and LDS accesses never alias, so the unrolled loads are hoisted
+  // ahead of the LDS writes
 #pragma unroll
-  for (int i = 0; i < IT; ++i) {
+  for (int i = 0; i < FULL; ++i) {
     const int e = tid + i * NTHR;
-    if (TOTAL % NTHR == 0 || e < TOTAL) {
-      const int k = e / VPR;
-      const int t = EPV * (e - k * VPR);
-      val[i] = *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
-    }
+    const int k = e / VPR;
+    const int t = EPV * (e - k * VPR);
+    *(float4*)&tile[tile_idx<R, T>(k, t)] = *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
   }
-#pragma unroll
-  for (int i = 0; i < IT; ++i) {
-    const int e = tid + i * NTHR;
-    if (TOTAL % NTHR == 0 || e < TOTAL) {
+  if constexpr (TOTAL % NTHR != 0) {
+    const int e = tid + FULL * NTHR;
+    if (e < TOTAL) {
       const int k = e / VPR;
       const int t = EPV * (e - k * VPR);
-      *(float4*)&tile[tile_idx<R, T>(k, t)] = val[i];
+      *(float4*)&tile[tile_idx<R, T>(k, t)] =
+          *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
     }
   }
 }
@@ -252,7 +285,7 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
 constexpr size_t lds_bytes() {
   constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
-  constexpr int rows = KIND == K_C2C ? M : M + 1;
+  constexpr int rows = (KIND == K_R2C || KIND == K_C2R) ? M + 1 : M;
   constexpr size_t tile = (LD == IO_TILE || ST == IO_TILE) ? (size_t)rows * T * sizeof(C<R>) : 0;
   return lines > tile ? lines : tile;
 }
@@ -263,6 +296,12 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   constexpr int TQ = T / LPW;
   constexpr int NTHR = TQ * L;
   static_assert(TQ * LPW == T, "LPW must divide T");
+  // inputs / outputs per line (the r2c output and c2r input carry M + 1)
+  constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
+  constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
+  constexpr bool GATHER = KIND == K_C2R || KIND == K_DCT2C || KIND == K_DCT3C;  // pre reads (k, M-k)
+  constexpr bool POST = KIND == K_R2C || KIND == K_DCT2C || KIND == K_DCT3C;    // post reads LDS
+  constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
   extern __shared__ __align__(16) char smem[];
   C<R>* lds = (C<R>*)smem;
   const int tid = threadIdx.x;
@@ -272,55 +311,66 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   C<R>* tile = lds;
   const C<R>* __restrict__ tw = (const C<R>*)a.tw;
   const C<R>* __restrict__ aux = (const C<R>*)a.aux;
-  const bool conj_io = KIND == K_C2C ? a.bwd != 0 : KIND == K_C2R;
+  // backward FFTs run as conj(FFT(conj x))
+  const bool conj_in = KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C);
+  const bool conj_out = KIND == K_C2C && a.bwd != 0;
   const R sc = a.do_scale ? (R)a.scale : R(1);
-  constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
 
   for (int64_t tl = blockIdx.x; tl < a.ntiles; tl += gridDim.x) {
     TileCtx c;
     tile_init(a, tl, c);
     C<R> v[LPW][NX];
 
-    // ---------------- load (+ c2r merge) ----------------
+    // ---------------- load and pre-process ----------------
     if constexpr (LD == IO_TILE) {
-      tile_load<R, T, KIND == K_C2R ? M + 1 : M, NTHR>(tile, a, c.in_base, tid);
+      tile_load<R, T, ROWS_IN, NTHR>(tile, a, c.in_base, tid);
       __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < LPW; ++q) {
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
-      if constexpr (KIND == K_C2R) {
+      if constexpr (GATHER) {
+        // element x_k of this line, from the tile or from the line's LDS slot
         if constexpr (LD == IO_LINE) {
+          if (q > 0) wave_sync();
           const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
           if (active)
-            for (int k = l; k <= M; k += L) line[pad8(k)] = in[k];
+            for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = in[k];
           wave_sync();
         }
-        // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
+        auto X = [&](int k) -> C<R> {
+          if constexpr (LD == IO_TILE) return tile[tile_idx<R, T>(k, t)];
+          else return line[pad8(k)];
+        };
 #pragma unroll
         for (int p = 0; p < P; ++p) {
           const int k = l + L * p;
-          C<R> A, B;
-          if constexpr (LD == IO_TILE) {
-            A = tile[tile_idx<R, T>(k, t)];
-            B = tile[tile_idx<R, T>(M - k, t)];
+          C<R> z;
+          if constexpr (KIND == K_C2R) {
+            // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
+            C<R> A = X(k), B = conj(X(M - k));
+            if (k == 0) { A.im = R(0); B.im = R(0); }
+            else if (a.adjoint) { A = scl(A, R(0.5)); B = scl(B, R(0.5)); }  // X_k / w_k
+            const C<R> E = A + B;
+            const C<R> O = (A - B) * conj(aux[k]);
+            z = E + mul_pi(O);
+          } else if constexpr (KIND == K_DCT2C) {
+            // Makhoul reorder: v_j = x_{2j} (j < ceil(M/2)), v_{M-1-j} = x_{2j+1}
+            z = X(2 * k < M ? 2 * k : 2 * M - 2 * k - 1);
           } else {
-            A = line[pad8(k)];
-            B = line[pad8(M - k)];
+            // V_k = w_4M^-k (x_k - i x_{M-k}), x_M = 0
+            const C<R> xk = X(k);
+            const C<R> xr = k == 0 ? mk(R(0), R(0)) : X(M - k);
+            z = conj(aux[k]) * (xk + mul_mi(xr));
           }
-          B = conj(B);
-          if (k == 0) { A.im = R(0); B.im = R(0); }
-          const C<R> E = A + B;
-          const C<R> O = (A - B) * conj(aux[k]);
-          v[q][p] = conj(E + mul_pi(O));   // backward FFT as conj(FFT(conj))
+          v[q][p] = conj_in ? conj(z) : z;
         }
-        if constexpr (LD == IO_LINE) wave_sync();
       } else if constexpr (LD == IO_TILE) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
           const C<R> x = tile[tile_idx<R, T>(l + L * p, t)];
-          v[q][p] = conj_io ? conj(x) : x;
+          v[q][p] = conj_in ? conj(x) : x;
         }
       } else {
         // contiguous line: complex elements, or real pairs (x[2m], x[2m+1]) for r2c
@@ -331,44 +381,63 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 #pragma unroll
           for (int p = 0; p < P; ++p) {
             const C<R> x = in[l + L * p];
-            v[q][p] = conj_io ? conj(x) : x;
+            v[q][p] = conj_in ? conj(x) : x;
           }
         }
       }
     }
     if constexpr (LD == IO_TILE) __syncthreads();   // line slots alias the tile
+    else if constexpr (GATHER) wave_sync();
 
-    // ---------------- FFT (+ r2c split, line stores) ----------------
+    // ---------------- FFT, post-process, line stores ----------------
 #pragma unroll
     for (int q = 0; q < LPW; ++q) {
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
       if (q > 0) wave_sync();
       Cfg<M>::template fft<R>(v[q], line, l, tw);
-      if constexpr (KIND == K_R2C) {
-        // X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
+      if constexpr (POST) {
+        // the transform lands in natural order in the line's LDS slot
         wave_sync();
 #pragma unroll
-        for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[q][p];
+        for (int p = 0; p < P; ++p)
+          line[pad8(l + L * p)] = KIND == K_DCT3C ? conj(v[q][p]) : v[q][p];
         wave_sync();
-        C<R> X[P + 1];
+        C<R> Y[NX];
 #pragma unroll
-        for (int p = 0; p <= P; ++p) {
+        for (int p = 0; p < NX; ++p) {
           const int k = p < P ? l + L * p : M;
-          const C<R> Z = line[pad8(k & (M - 1))];
-          const C<R> Zc = conj(line[pad8((M - k) & (M - 1))]);
-          const C<R> E = scl(Z + Zc, R(0.5));
-          const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
-          X[p] = E + aux[k] * O;
+          if constexpr (KIND == K_R2C) {
+            // X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
+            const C<R> Z = line[pad8(k == M ? 0 : k)];
+            const C<R> Zc = conj(line[pad8(k == 0 ? 0 : M - k)]);
+            const C<R> E = scl(Z + Zc, R(0.5));
+            const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
+            Y[p] = E + aux[k] * O;
+            if (a.adjoint && k != 0 && k != M) Y[p] = scl(Y[p], R(2));    // w_k X_k
+          } else if constexpr (KIND == K_DCT2C) {
+            // split V into the transforms of the real and imaginary parts,
+            // then y_k = 2 Re(w_4M^k V_k) for each
+            const C<R> Vk = line[pad8(k)];
+            const C<R> Vn = conj(line[pad8(k == 0 ? 0 : M - k)]);
+            const C<R> Va = scl(Vk + Vn, R(0.5));
+            const C<R> Vb = scl(mul_mi(Vk - Vn), R(0.5));
+            const C<R> w = aux[k];
+            Y[p] = scl(mk(w.re * Va.re - w.im * Va.im, w.re * Vb.re - w.im * Vb.im), R(2) * sc);
+          } else {
+            // inverse Makhoul reorder: y_{2m} = v_m, y_{2m+1} = v_{M-1-m}
+            Y[p] = scl(line[pad8((k & 1) ? M - 1 - (k >> 1) : (k >> 1))], sc);
+          }
         }
 #pragma unroll
-        for (int p = 0; p <= P; ++p) v[q][p] = X[p];
+        for (int p = 0; p < NX; ++p) v[q][p] = Y[p];
         if constexpr (ST == IO_LINE) {
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) out[l + L * p] = X[p];
-            if (l == 0) out[M] = X[P];
+            for (int p = 0; p < P; ++p) out[l + L * p] = Y[p];
+            if constexpr (KIND == K_R2C)
+              if (l == 0) out[M] = Y[P];
           }
         }
       } else if constexpr (KIND == K_C2R) {
@@ -378,13 +447,14 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 #pragma unroll
           for (int p = 0; p < P; ++p) out[l + L * p] = scl(conj(v[q][p]), sc);
         }
-      } else if constexpr (ST == IO_LINE) {
-        C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
-        if (active) {
+      } else {
 #pragma unroll
-          for (int p = 0; p < P; ++p) {
-            const C<R> y = conj_io ? conj(v[q][p]) : v[q][p];
-            out[l + L * p] = scl(y, sc);
+        for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], sc);
+        if constexpr (ST == IO_LINE) {
+          C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+          if (active) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p];
           }
         }
       }
@@ -396,20 +466,13 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 #pragma unroll
       for (int q = 0; q < LPW; ++q) {
         const int t = q * TQ + g;
-        if constexpr (KIND == K_R2C) {
 #pragma unroll
-          for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = v[q][p];
+        for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = v[q][p];
+        if constexpr (KIND == K_R2C)
           if (l == 0) tile[tile_idx<R, T>(M, t)] = v[q][P];
-        } else {
-#pragma unroll
-          for (int p = 0; p < P; ++p) {
-            const C<R> y = conj_io ? conj(v[q][p]) : v[q][p];
-            tile[tile_idx<R, T>(l + L * p, t)] = scl(y, sc);
-          }
-        }
       }
       __syncthreads();
-      tile_store<R, T, KIND == K_R2C ? M + 1 : M, NTHR>(tile, a, c.out_base, tid);
+      tile_store<R, T, ROWS_OUT, NTHR>(tile, a, c.out_base, tid);
     }
     __syncthreads();
   }
@@ -448,6 +511,20 @@ static void add(std::vector<Entry>& v) {
 // Candidates per (precision, length, kind, load, store).  The first of each
 // group is the default when autotuning is off; the autotuner times them all.
 template <typename R, int M>
+static void add_dct(std::vector<Entry>& v) {
+  constexpr int TL = 256 / Cfg<M>::L;
+  constexpr int TT = sizeof(R) == 4 ? 16 : 8;
+  add<R, M, TL, 1, K_DCT2C, IO_LINE, IO_LINE>(v);
+  add<R, M, TL, 1, K_DCT3C, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DCT2C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 2, K_DCT2C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DCT3C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 2, K_DCT3C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DCT2C, IO_TILE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DCT3C, IO_TILE, IO_LINE>(v);
+}
+
+template <typename R, int M>
 static void add_size(std::vector<Entry>& v) {
   constexpr int TL = 256 / Cfg<M>::L;       // contiguous passes: 256 threads
   constexpr int TT = sizeof(R) == 4 ? 16 : 8;  // transposed runs of 128 bytes
@@ -469,8 +546,10 @@ static void add_size(std::vector<Entry>& v) {
     if constexpr (sizeof(R) == 4) {
       add<R, M, 32, 4, K_C2C, IO_LINE, IO_TILE>(v);
       add<R, M, 32, 4, K_C2C, IO_TILE, IO_LINE>(v);
-      add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE>(v);
-      add<R, M, 32, 2, K_C2R, IO_TILE, IO_LINE>(v);
+      if constexpr (M <= 256) {   // larger lines spill at 128 VGPRs
+        add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE>(v);
+        add<R, M, 32, 2, K_C2R, IO_TILE, IO_LINE>(v);
+      }
     }
   } else {
     add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE>(v);
@@ -490,6 +569,19 @@ static const std::vector<Entry>& registry() {
     add_size<float, 256>(r);
     add_size<float, 512>(r);
     add_size<float, 1024>(r);
+    add_size<float, 32>(r);
+    add_size<float, 48>(r);
+    add_size<float, 96>(r);
+    add_size<float, 160>(r);
+    add_size<float, 192>(r);
+    add_size<float, 320>(r);
+    add_size<float, 384>(r);
+    add_dct<float, 256>(r);
+    add_dct<float, 128>(r);
+    add_dct<float, 64>(r);
+    add_dct<float, 512>(r);
+    add_dct<double, 256>(r);
+    add_dct<double, 64>(r);
     add_size<double, 64>(r);
     add_size<double, 128>(r);
     add_size<double, 256>(r);
@@ -518,6 +610,8 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp) {
   if (a.kind == L_C2C) kind = K_C2C;
   else if (a.kind == L_R2C_HALF) kind = K_R2C;
   else if (a.kind == L_C2R_HALF) kind = K_C2R;
+  else if (a.kind == L_DCT2 && a.r2r_cplx) kind = K_DCT2C;
+  else if (a.kind == L_DCT3 && a.r2r_cplx) kind = K_DCT3C;
   else return out;
   const int rb = p.real_type == PFFT_F32 ? 4 : 8;
   const int M = (int)a.M;
@@ -600,7 +694,7 @@ const char* fast_name(int variant) {
   const std::vector<Entry>& reg = registry();
   if (variant < 1 || variant > (int)reg.size()) return "generic";
   const Entry& e = reg[variant - 1];
-  static const char* kinds[3] = {"c2c", "r2c", "c2r"};
+  static const char* kinds[5] = {"c2c", "r2c", "c2r", "dct2c", "dct3c"};
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +

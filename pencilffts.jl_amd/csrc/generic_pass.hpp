@@ -108,6 +108,7 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         cpx<R> A = raw_cplx(a, raw, w, j);
         cpx<R> B = conj(raw_cplx(a, raw, w, M - j));
         if (j == 0) { A.im = zero; B.im = zero; }  // Im of DC and Nyquist ignored
+        else if (a.adjoint) { A = scl(A, R(0.5)); B = scl(B, R(0.5)); }  // X_k / w_k
         const cpx<R> E = A + B;
         const cpx<R> O = (A - B) * conj(aux[j]);
         v = E + mul_pi(O);
@@ -117,6 +118,7 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         if (j == 0) v = mk(raw_cplx(a, raw, w, 0).re, zero);
         else if (2 * (int64_t)j < n) v = raw_cplx(a, raw, w, j);
         else v = conj(raw_cplx(a, raw, w, n - j));
+        if (a.adjoint && j != 0) v = scl(v, R(0.5));  // X_k / w_k
         break;
       }
       case L_DCT1:
@@ -338,9 +340,15 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
   }
 }
 
+// Hermitian weight of bin k of an r2c of length n: 1 for the bins stored
+// once (DC, and Nyquist for even n), 2 for the others.  The adjoint of the
+// r2c multiplies by it; the adjoint of the c2r divides by it.
+PFFT_HD int herm_weight(int64_t k, int64_t n) { return (k == 0 || 2 * k == n) ? 1 : 2; }
+
 template <typename R>
 PFFT_HD cpx<R> post_cplx(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
   const cpx<R>* L = W + w * a.Mp;
+  cpx<R> y;
   if (a.kind == L_R2C_HALF) {
     const cpx<R>* aux = (const cpx<R>*)a.aux;
     const int64_t M = a.M;
@@ -348,9 +356,13 @@ PFFT_HD cpx<R> post_cplx(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t
     const cpx<R> Zc = conj(L[k == 0 ? 0 : M - k]);
     const cpx<R> E = scl(Z + Zc, R(0.5));
     const cpx<R> O = scl(mul_mi(Z - Zc), R(0.5));
-    return E + aux[k] * O;
+    y = E + aux[k] * O;
+  } else {
+    y = L[k];  // C2C, R2C_FULL
   }
-  return L[k];  // C2C, R2C_FULL
+  if (a.adjoint && (a.kind == L_R2C_HALF || a.kind == L_R2C_FULL))
+    y = scl(y, R(herm_weight(k, a.n)));
+  return y;
 }
 
 template <typename R>

@@ -272,11 +272,17 @@ static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
 
 int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
                    hipStream_t stream) {
-  if (dir != PFFT_FORWARD && dir != PFFT_BACKWARD && dir != PFFT_BACKWARD_SCALED)
+  if (dir < PFFT_FORWARD || dir > PFFT_ADJOINT_BACKWARD)
     return fail(PFFT_ERR_ARGUMENT, "unknown direction " + std::to_string(dir));
   if (!in || !out) return fail(PFFT_ERR_ARGUMENT, "null array pointer");
-  const size_t nin = dir == PFFT_FORWARD ? p.in_bytes : p.out_bytes;
-  const size_t nout = dir == PFFT_FORWARD ? p.out_bytes : p.in_bytes;
+  const bool adjoint = dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD;
+  if (adjoint && !p.adjoint_ok)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "adjoints are provided for FFT/BFFT/RFFT/BRFFT/NoTransform plans only");
+  // data flows input-shaped -> output-shaped for mul! and the adjoint of bmul!
+  const bool fwd_shape = dir == PFFT_FORWARD || dir == PFFT_ADJOINT_BACKWARD;
+  const size_t nin = fwd_shape ? p.in_bytes : p.out_bytes;
+  const size_t nout = fwd_shape ? p.out_bytes : p.in_bytes;
   if (overlaps(in, nin, out, nout))
     return fail(PFFT_ERR_ARGUMENT, "out-of-place plan applied to aliased data");
   int r = device_prepare(p);
@@ -301,7 +307,9 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
       return fail(PFFT_ERR_ARGUMENT, "workspace overlaps the input or output array");
     }
   }
-  const std::vector<PassPlan>& passes = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+  // the adjoint of mul! runs the backward passes, the adjoint of bmul! the
+  // forward ones, each with the Hermitian weights of its r2c/c2r stage
+  const std::vector<PassPlan>& passes = fwd_shape ? p.fwd : p.bwd;
   void* bufs[3] = {nullptr, wk, wk ? wk + p.work0_bytes : nullptr};
   const int np = (int)passes.size();
   if (d->timing) HIP_TRY(hipEventRecord(d->ev[0], stream));
@@ -316,6 +324,7 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
       a.do_scale = 1;
       a.scale = 1.0 / p.scale;
     }
+    a.adjoint = adjoint ? 1 : 0;
     if (pp.variant == 0) {
       if (p.real_type == PFFT_F32)
         hipLaunchKernelGGL(k_generic<float>, dim3((unsigned)pp.grid), dim3(pp.threads),

@@ -529,6 +529,9 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
     }
   }
   p.inplace = ip0;
+  p.adjoint_ok = true;
+  for (int d = 0; d < N; ++d)
+    if (is_r2r(kind_base(kinds[d]))) p.adjoint_ok = false;
   for (int d = 0; d < N; ++d) {
     int r = length_output(kinds[d], dims[d], &p.size_out[d]);
     if (r) return r;

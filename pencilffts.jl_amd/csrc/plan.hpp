@@ -80,6 +80,7 @@ struct Plan {
   int32_t real_type = 1;     // PFFT_F32 / PFFT_F64
   bool permute = true;
   bool inplace = false;
+  bool adjoint_ok = true;  // no R2R stage: the adjoint directions are defined
   uint32_t flags = 0;
   double scale = 1.0;
   int32_t elt_in = 0, elt_out = 0;

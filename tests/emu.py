@@ -43,7 +43,7 @@ class Emulator:
         try:
             bi, bo = C.c_size_t(), C.c_size_t()
             self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
-            nin, nout = (bi.value, bo.value) if direction == 0 else (bo.value, bi.value)
+            nin, nout = (bi.value, bo.value) if direction in (0, 4) else (bo.value, bi.value)
             src = np.ascontiguousarray(x).reshape(-1)
             assert src.nbytes == nin, (src.nbytes, nin)
             out = np.zeros(nout, dtype=np.uint8)

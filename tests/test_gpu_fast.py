@@ -44,3 +44,52 @@ def test_fast_kernels_selected(pf, gpu):
     bwd = plan.kernels(2)
     print(fwd, bwd)
     assert all(k.startswith("k_fast") for k in fwd + bwd), (fwd, bwd)
+
+
+MIXED_SHAPES = [(384, 96, 64), (192, 384, 32), (320, 160, 64), (96, 192, 48), (64, 320, 96)]
+
+
+@pytest.mark.parametrize("permute", [True, False], ids=["perm", "noperm"])
+@pytest.mark.parametrize("tname", ["RFFT", "FFT", "MIXED"])
+@pytest.mark.parametrize("dims", MIXED_SHAPES, ids=lambda d: "x".join(map(str, d)))
+def test_fast_mixed_radix(pf, gpu, dims, tname, permute):
+    """Lengths 3*2^k and 5*2^k (the mixed and batched workloads' sizes) and the
+    complex-data REDFT10/REDFT01 passes, Float32."""
+    T = pf.Transforms
+    if tname == "MIXED":
+        trs = (T.RFFT(), T.FFT(), T.R2R(T.REDFT10))
+    else:
+        trs = (getattr(T, tname)(),)
+    err_f, err_b, meta = run_case(pf, dims, trs, 0, permute=permute, seed=5)
+    tol = 5e-7 * np.log2(float(np.prod(dims)))
+    print(dims, tname, permute, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= tol and err_b <= tol
+
+
+def test_mixed_workload_full_size(pf, gpu):
+    """The mixed workload at its full size: (RFFT, FFT, R2R(REDFT10)) on a
+    Float32 384x384x256 field, inverse through (BRFFT, BFFT, REDFT01), vs the
+    FP64 oracle; every pass on the specialised kernels."""
+    from oracle import pipeline as OP
+    from oracle import plan as OM
+    T = pf.Transforms
+    dims = (384, 384, 256)
+    trs = (T.RFFT(), T.FFT(), T.R2R(T.REDFT10))
+    plan = pf.PencilFFTPlan(dims, trs, (1, 1), None, T.Float32)
+    ks = plan.kernels(0) + plan.kernels(2)
+    print(ks)
+    assert all(k.startswith("k_fast") for k in ks), ks
+    u = pf.allocate_input(plan)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    u.parent.normal_(generator=g)
+    v = plan * u
+    w = plan.ldiv(v)
+    torch.cuda.synchronize()
+    x = u.parent.cpu().numpy().astype(np.float64).reshape(-1)
+    err_b = OP.rel_l2(w.parent.cpu().numpy().reshape(-1), x)
+    meta = OM.PlanMeta(dims, ("RFFT", "FFT", "R2R_REDFT10"), "Float64")
+    ref = OP.forward(meta, OP.from_image(x, dims), workers=16)
+    err_f = OP.rel_l2(v.parent.cpu().numpy().reshape(-1), OP.memory_image(ref))
+    print("mixed 384x384x256: fwd %.2e rt %.2e" % (err_f, err_b))
+    tol = 5e-7 * np.log2(float(np.prod(dims)))
+    assert err_f <= tol and err_b <= tol
