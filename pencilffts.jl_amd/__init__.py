@@ -6,12 +6,14 @@ hand-written HIP kernels for gfx950 through the C ABI of include/pfft.h.
 """
 from . import _lib
 from . import transforms as Transforms
+from . import autodiff, batched
 from ._lib import ArgumentError, DeviceError, PfftError, UnsupportedError
 from .plans import (Pencil, PencilArray, PencilFFTPlan, Val, allocate_input, allocate_output,
                     bmul_, execute_many, ldiv_, mul_, scale_factor)
 
 __all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "Pencil", "allocate_input",
            "allocate_output", "mul_", "ldiv_", "bmul_", "scale_factor", "Val",
-           "execute_many", "ArgumentError", "UnsupportedError", "DeviceError", "PfftError"]
+           "execute_many", "ArgumentError", "UnsupportedError", "DeviceError", "PfftError",
+           "autodiff", "batched"]
 
 __version__ = _lib.lib.pfft_version().decode()

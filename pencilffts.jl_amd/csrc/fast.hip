@@ -185,6 +185,185 @@ template <> struct Cfg<1024> {
   }
 };
 
+// ---- radix-16 stages on MFMA (1024-point lines) ------------------------------------
+// A radix-16 butterfly is the 16x16 DFT matrix times the twiddled inputs; in
+// real form a 32x32 matrix G times [Re x; Im x].  One wavefront owns a line's
+// 64 butterflies as two 32-column blocks of v_mfma_f32_32x32x8 (four K-chunks
+// of 8).  Lane (j = l % 32, h = l / 32) holds points r in
+// {4h..4h+3, 8+4h..11+4h} of butterfly j -- both the B-operand layout of the
+// instruction and the layout of its accumulators, so inputs and outputs of a
+// lane are the same eight points.
+//   MF = 1  split-FP16: x = x_hi + x_lo and G = G_hi + G_lo in FP16,
+//           G_hi x_hi + G_hi x_lo + G_lo x_hi accumulated in FP32; the line
+//           is scaled by a power of two (exact) so that |x| < 2^14.
+//   MF = 2  BF16x3: three-term BF16 splits, the six products whose order is at
+//           most the third term, accumulated in FP32.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__constant__ float kCos16[16] = {
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+__constant__ float kSin16[16] = {
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f};
+
+// G[i][s]: rows (Re y_q, Im y_q), columns (Re x_r, Im x_r), y = F x with
+// F_qr = exp(-2 pi i q r / 16) = c - i sn:  Re y = c Re x + sn Im x,
+// Im y = -sn Re x + c Im x.
+__device__ __forceinline__ float g16(int i, int s) {
+  const int m = ((i & 15) * (s & 15)) & 15;
+  const float c = kCos16[m], sn = kSin16[m];
+  if (i < 16) return s < 16 ? c : sn;
+  return s < 16 ? -sn : c;
+}
+
+__device__ __forceinline__ short f2bf(float x) {   // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) {
+  return __uint_as_float(((unsigned)(unsigned short)b) << 16);
+}
+
+template <int MF>
+struct MfmaA {   // this lane's A operands, four K-chunks
+  f16x4 hi[4], lo[4];
+  bf16x4 b1[4], b2[4], b3[4];
+  __device__ void init(int l) {
+    const int i = l & 31, h = l >> 5;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float g = g16(i, 8 * c + 4 * h + t);
+        if constexpr (MF == 1) {
+          const _Float16 gh = (_Float16)g;
+          hi[c][t] = gh;
+          lo[c][t] = (_Float16)(g - (float)gh);
+        } else {
+          const short a1 = f2bf(g);
+          const float r1 = g - bf2f(a1);
+          const short a2 = f2bf(r1);
+          b1[c][t] = a1;
+          b2[c][t] = a2;
+          b3[c][t] = f2bf(r1 - bf2f(a2));
+        }
+      }
+  }
+};
+
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+template <int MF>
+__device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float>* x) {
+  // x[t]: the lane's eight points; K-chunk c holds s = 8c + 4h + t, i.e.
+  // c = 0: Re x[0..3], 1: Re x[4..7], 2: Im x[0..3], 3: Im x[4..7]
+  f32x16 d = {};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[t] = c < 2 ? x[4 * c + t].re : x[4 * (c - 2) + t].im;
+    if constexpr (MF == 1) {
+      f16x4 bh, bl;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bh[t] = (_Float16)b[t];
+        bl[t] = (_Float16)(b[t] - (float)bh[t]);
+      }
+      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.hi[c], bh, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.hi[c], bl, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.lo[c], bh, d, 0, 0, 0);
+    } else {
+      bf16x4 x1, x2, x3;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        x1[t] = f2bf(b[t]);
+        const float r1 = b[t] - bf2f(x1[t]);
+        x2[t] = f2bf(r1);
+        x3[t] = f2bf(r1 - bf2f(x2[t]));
+      }
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x2, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b2[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x3, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b3[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b2[c], x2, d, 0, 0, 0);
+    }
+  }
+  return d;
+}
+
+// One radix-16 Stockham stage of a 1024-point line on MFMA, LDS to LDS.
+template <int MF, int NS>
+__device__ __forceinline__ void stage16_mfma(C<float>* line, int l, const C<float>* __restrict__ tw,
+                                             const MfmaA<MF>& A) {
+  constexpr int M = 1024, NB = M / 16, STEP = M / (NS * 16);
+  const int jj = l & 31, h = l >> 5;
+  C<float> x[2][8];
+#pragma unroll
+  for (int be = 0; be < 2; ++be) {
+    const int b = 32 * be + jj;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int r = t < 4 ? 4 * h + t : 8 + 4 * h + (t - 4);
+      C<float> v = line[pad8(b + NB * r)];
+      if constexpr (NS > 1) v = v * tw[r * (b % NS) * STEP];
+      x[be][t] = v;
+    }
+  }
+  int e = 0;
+  if constexpr (MF == 1) {
+    float mx = 0.f;
+#pragma unroll
+    for (int be = 0; be < 2; ++be)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) mx = fmaxf(mx, fmaxf(fabsf(x[be][t].re), fabsf(x[be][t].im)));
+    mx = wave_max(mx);
+    e = mx > 0.f ? 13 - ilogbf(mx) : 0;   // |x| 2^e < 2^14
+#pragma unroll
+    for (int be = 0; be < 2; ++be)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) x[be][t] = mk(ldexpf(x[be][t].re, e), ldexpf(x[be][t].im, e));
+  }
+#pragma unroll
+  for (int be = 0; be < 2; ++be) {
+    const f32x16 d = mfma_block<MF>(A, x[be]);
+    const int b = 32 * be + jj;
+    const int base = (b / NS) * NS * 16 + (b % NS);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int q = t < 4 ? 4 * h + t : 8 + 4 * h + (t - 4);
+      line[pad8(base + q * NS)] = mk(ldexpf(d[t], -e), ldexpf(d[8 + t], -e));
+    }
+  }
+}
+
+// 1024 = 16 (MFMA) x 16 (MFMA) x 4 (registers)
+template <int MF>
+__device__ __forceinline__ void fft1024_mfma(C<float>* v, C<float>* line, int l,
+                                             const C<float>* tw, const MfmaA<MF>& A) {
+#pragma unroll
+  for (int p = 0; p < 16; ++p) line[pad8(l + 64 * p)] = v[p];
+  wave_sync();
+  stage16_mfma<MF, 1>(line, l, tw, A);
+  wave_sync();
+  stage16_mfma<MF, 16>(line, l, tw, A);
+  wave_sync();
+  stage<float, 1024, 64, 16, 4, 256, false, true>(v, line, l, tw);
+}
+
 // mixed radix: P is a multiple of every radix of the schedule
 #define PFFT_CFG(MM, LL, PP, ...)                                        \
   template <> struct Cfg<MM> {                                           \
@@ -290,7 +469,7 @@ constexpr size_t lds_bytes() {
   return lines > tile ? lines : tile;
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF>
 __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M>::L, P = Cfg<M>::P;
   constexpr int TQ = T / LPW;
@@ -315,6 +494,9 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   const bool conj_in = KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C);
   const bool conj_out = KIND == K_C2C && a.bwd != 0;
   const R sc = a.do_scale ? (R)a.scale : R(1);
+  static_assert(MF == 0 || (M == 1024 && sizeof(R) == 4), "MFMA stages: 1024-point f32 lines");
+  MfmaA<MF == 0 ? 1 : MF> mfA;
+  if constexpr (MF != 0) mfA.init(l % 64);
 
   for (int64_t tl = blockIdx.x; tl < a.ntiles; tl += gridDim.x) {
     TileCtx c;
@@ -395,7 +577,9 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
       if (q > 0) wave_sync();
-      Cfg<M>::template fft<R>(v[q], line, l, tw);
+      if constexpr (MF != 0) fft1024_mfma<MF>((C<float>*)v[q], (C<float>*)line, l,
+                                              (const C<float>*)tw, mfA);
+      else Cfg<M>::template fft<R>(v[q], line, l, tw);
       if constexpr (POST) {
         // the transform lands in natural order in the line's LDS slot
         wave_sync();
@@ -480,18 +664,18 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF>), g, b, lds, s, a);
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -501,15 +685,26 @@ static void add(std::vector<Entry>& v) {
   e.st = ST;
   e.T = T;
   e.lpw = LPW;
+  e.mf = MF;
   e.threads = T / LPW * Cfg<M>::L;
   e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST>;
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF>;
   v.push_back(e);
 }
 
 // Candidates per (precision, length, kind, load, store).  The first of each
 // group is the default when autotuning is off; the autotuner times them all.
+// c2c 1024-point passes with the radix-16 stages on MFMA (precision flags)
+template <int MF>
+static void add_mfma(std::vector<Entry>& v) {
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, MF>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, MF>(v);
+  add<float, 1024, 8, 2, K_C2C, IO_LINE, IO_TILE, MF>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_TILE, IO_LINE, MF>(v);
+  add<float, 1024, 8, 2, K_C2C, IO_TILE, IO_LINE, MF>(v);
+}
+
 template <typename R, int M>
 static void add_dct(std::vector<Entry>& v) {
   constexpr int TL = 256 / Cfg<M>::L;
@@ -569,6 +764,8 @@ static const std::vector<Entry>& registry() {
     add_size<float, 256>(r);
     add_size<float, 512>(r);
     add_size<float, 1024>(r);
+    add_mfma<1>(r);
+    add_mfma<2>(r);
     add_size<float, 32>(r);
     add_size<float, 48>(r);
     add_size<float, 96>(r);
@@ -626,10 +823,17 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp) {
   // real pairs of r2c input / c2r output need even line offsets
   if (kind == K_R2C && !all_even(a.lsin, 0, a.D)) return out;
   if (kind == K_C2R && !all_even(a.lsout, 0, a.D)) return out;
+  // precision flags put the radix-16 stages of 1024-point c2c lines on MFMA
+  int want_mf = 0;
+  if (kind == K_C2C && M == 1024 && rb == 4) {
+    if (p.flags & PFFT_FLAG_MFMA_SPLIT16) want_mf = 1;
+    else if (p.flags & PFFT_FLAG_MFMA_BF16X3) want_mf = 2;
+  }
   const std::vector<Entry>& reg = registry();
   for (size_t i = 0; i < reg.size(); ++i) {
     const Entry& x = reg[i];
     if (x.real_bytes != rb || x.M != M || x.kind != kind || x.ld != ld || x.st != st) continue;
+    if (x.mf != want_mf) continue;
     if (x.lds > 160 * 1024) continue;
     if (ld == IO_TILE || st == IO_TILE) {
       // transposed tiles: full tiles, 16-byte aligned runs
@@ -698,7 +902,8 @@ const char* fast_name(int variant) {
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
-      std::to_string(e.lpw) + ">";
+      std::to_string(e.lpw) + (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
+      ">";
   return s.c_str();
 }
 

@@ -229,6 +229,12 @@ class PencilFFTPlan:
         L.check(L.lib.pfft_plan_kernels(self._h, direction, buf, 4096))
         return buf.value.decode().split(";")
 
+    def prepare(self):
+        """Upload tables and pick kernels now (pfft_plan_prepare) rather than at
+        the first execute -- needed before capturing executes in a graph."""
+        L.check(L.lib.pfft_plan_prepare(self._h, None))
+        self._workspace()
+
     def set_timing(self, on=True):
         L.check(L.lib.pfft_plan_set_timing(self._h, int(on)))
 
