@@ -5,10 +5,21 @@ A step is one pair, as in the reference's benchmark_rfft
 (benchmarks/benchmarks.jl:236-245): mul!(v, plan, u) followed by
 ldiv!(u, plan, v), inputs resident in HBM.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--dims 512]
+Beside the headline line's required fields, "extras" holds the other
+workloads of the benchmark set, each timed the same way (HIP events around
+K pairs on the launch stream):
+  * 512^3 ComplexF32 FFT + BFFT pair, 256^3 Float64 RFFT pair;
+  * (RFFT, FFT, R2R REDFT10) on a Float32 384x384x256 field;
+  * the precision form: 1024^3 ComplexF32 FFT pair with plain FP32 radix
+    stages, split-FP16 MFMA and BF16x3 MFMA radix-16 stages, each with its
+    relative L2 error against the Float64 plan of the same input;
+  * 12 uneven Float32 vector fields, loss + gradient in one hipGraph replay.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -17,6 +28,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X datasheet
+METRIC = "512³ r2c+c2r FFT: ms/pair, % HBM peak; split-FP16 MFMA vs FP32 at 1e-6 rel"
+ELT_BYTES = {0: 4, 1: 8, 2: 8, 3: 16}    # F32, F64, C32, C64
 
 
 def parse():
@@ -28,13 +41,21 @@ def parse():
     ap.add_argument("--transform", default="RFFT", choices=["RFFT", "FFT"])
     ap.add_argument("--real", default="Float32", choices=["Float32", "Float64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with two 1 GiB device copies: the known-size reference that "
+                         "scales HBM byte counters (rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
-def algorithmic_bytes(pass_elems_in, pass_elems_out, bytes_in, bytes_out):
-    return pass_elems_in * bytes_in + pass_elems_out * bytes_out
+def host_workers():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def cpu_baseline(dims, transform, real, reps):
@@ -46,7 +67,7 @@ def cpu_baseline(dims, transform, real, reps):
     from oracle import pipeline as OP
     from oracle import plan as OM
 
-    workers = os.cpu_count() or 1
+    workers = host_workers()
     meta = OM.PlanMeta(dims, (transform,), "Float64")
     rng = np.random.default_rng(42)
     if transform == "RFFT":
@@ -81,6 +102,219 @@ def cpu_baseline(dims, transform, real, reps):
                          workers, reps, cpu)}
 
 
+# ---------------------------------------------------------------- helpers
+
+def pass_bytes(pf, plan):
+    """Algorithmic HBM bytes of each forward pass (read its input pencil once,
+    write its output pencil once), in launch order; the backward passes move
+    the same bytes in reverse order."""
+    T = pf.Transforms
+    ext = math.prod(plan.extra_dims) if plan.extra_dims else 1
+    et = plan.eltype_input
+    out = []
+    for n in range(1, plan.N + 1):
+        _, di, do, _ = plan.stage(n)
+        et_out = T.eltype_output(plan.transforms[n - 1], et)
+        out.append(ext * (math.prod(di) * ELT_BYTES[et] + math.prod(do) * ELT_BYTES[et_out]))
+        et = et_out
+    return out
+
+
+def fill(torch, a, gen, kind="uniform"):
+    p = a.parent
+    if p.is_complex():
+        rt = torch.float32 if p.dtype == torch.complex64 else torch.float64
+        p.copy_(torch.complex(torch.randn(p.shape, generator=gen, device=p.device, dtype=rt),
+                              torch.randn(p.shape, generator=gen, device=p.device, dtype=rt)))
+    elif kind == "uniform":
+        p.uniform_(-1, 1, generator=gen)
+    else:
+        p.normal_(generator=gen)
+
+
+def time_pairs(torch, pf, plan, u, v, steps, warmup):
+    """ms per pair (mul! then ldiv!), HIP events on the launch stream."""
+    for _ in range(warmup):
+        pf.mul_(v, plan, u)
+        pf.ldiv_(u, plan, v)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        pf.mul_(v, plan, u)
+        pf.ldiv_(u, plan, v)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def pass_times(pf, plan, u, v, nrep):
+    """Per-pass kernel ms of a forward and a backward (library HIP events)."""
+    plan.set_timing(True)
+    fwd = [0.0] * plan.N
+    bwd = [0.0] * plan.N
+    for _ in range(nrep):
+        pf.mul_(v, plan, u)
+        for i, t in enumerate(plan.stage_times()):
+            fwd[i] += t / nrep
+        pf.ldiv_(u, plan, v)
+        for i, t in enumerate(plan.stage_times()):
+            bwd[i] += t / nrep
+    plan.set_timing(False)
+    return fwd, bwd
+
+
+def pair_entry(torch, pf, plan, u, v, steps, warmup, label):
+    ms = time_pairs(torch, pf, plan, u, v, steps, warmup)
+    pb = pass_bytes(pf, plan)
+    pair = 2 * sum(pb)
+    return {"workload": label, "ms_per_pair": round(ms, 4),
+            "hbm_frac": round(pair / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pair_algorithmic_bytes": pair,
+            "kernels": plan.kernels(0) + plan.kernels(2)}
+
+
+def rel_l2_chunked(torch, a, b, chunks=16):
+    """||a - b|| / ||b|| in float64 on the GPU, over the leading dim in chunks."""
+    num = 0.0
+    den = 0.0
+    n = a.shape[0]
+    step = max(1, (n + chunks - 1) // chunks)
+    for i in range(0, n, step):
+        x = a[i:i + step].to(b.dtype)
+        y = b[i:i + step]
+        num += float(torch.sum(torch.abs(x - y) ** 2))
+        den += float(torch.sum(torch.abs(y) ** 2))
+    return math.sqrt(num / den)
+
+
+# ----------------------------------------------------------------- extras
+
+def extra_plain(torch, pf, steps, warmup):
+    T = pf.Transforms
+    res = {}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    for key, dims, tr, rt, label in (
+            ("fft_512_c32", (512,) * 3, T.FFT(), T.Float32, "512^3 ComplexF32 FFT+BFFT pair"),
+            ("rfft_256_f64", (256,) * 3, T.RFFT(), T.Float64, "256^3 Float64 RFFT+BRFFT pair"),
+            ("mixed_384x384x256_f32", (384, 384, 256),
+             (T.RFFT(), T.FFT(), T.R2R(T.REDFT10)), T.Float32,
+             "384x384x256 Float32 (RFFT, FFT, REDFT10) pair")):
+        plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, rt)
+        u = pf.allocate_input(plan)
+        v = pf.allocate_output(plan)
+        fill(torch, u, g)
+        res[key] = pair_entry(torch, pf, plan, u, v, steps, warmup, label)
+        del plan, u, v
+        torch.cuda.empty_cache()
+    return res
+
+
+def extra_precision(torch, pf, steps, warmup, edge=1024):
+    """1024^3 ComplexF32 FFT pair: plain FP32 radix stages vs radix-16 MFMA
+    stages (split-FP16 hi+lo, BF16x3), error vs the Float64 plan."""
+    T = pf.Transforms
+    dims = (edge,) * 3
+    g = torch.Generator(device="cuda").manual_seed(11)
+    p64 = pf.PencilFFTPlan(dims, T.FFT(), (1, 1), None, T.Float64)
+    u64 = pf.allocate_input(p64)
+    fill(torch, u64, g)
+    # the FP32 input is the rounded FP64 input; the reference spectrum is the
+    # FP64 transform of exactly that rounded input
+    u32_data = u64.parent.to(torch.complex64)
+    u64.parent.copy_(u32_data)
+    v64 = pf.allocate_output(p64)
+    pf.mul_(v64, p64, u64)
+    torch.cuda.synchronize()
+    del p64
+    ref = v64.parent
+    res = {"workload": "%d^3 ComplexF32 FFT+BFFT pair" % edge, "input": "re/im iid N(0,1)",
+           "error_reference": "Float64 plan on the same (rounded) input"}
+    v = u = None
+    for name, flags in (("fp32", 0), ("split16", 4), ("bf16x3", 8)):
+        plan = pf.PencilFFTPlan(dims, T.FFT(), (1, 1), None, T.Float32, flags=flags)
+        if u is None:
+            u = pf.allocate_input(plan)
+            v = pf.allocate_output(plan)
+        u.parent.copy_(u32_data)
+        pf.mul_(v, plan, u)
+        torch.cuda.synchronize()
+        err_f = rel_l2_chunked(torch, v.parent, ref)
+        pf.ldiv_(u, plan, v)
+        torch.cuda.synchronize()
+        err_rt = rel_l2_chunked(torch, u.parent, u64.parent)
+        ent = pair_entry(torch, pf, plan, u, v, steps, warmup, name)
+        ent.pop("workload")
+        ent["rel_l2_forward_vs_f64"] = float("%.3e" % err_f)
+        ent["rel_l2_roundtrip"] = float("%.3e" % err_rt)
+        res[name] = ent
+        del plan
+        torch.cuda.empty_cache()
+    f, s = res["fp32"]["ms_per_pair"], res["split16"]["ms_per_pair"]
+    res["split16_speedup_vs_fp32"] = round(f / s, 3)
+    del u, v, u64, v64, ref, u32_data
+    torch.cuda.empty_cache()
+    return res
+
+
+def extra_batched(torch, pf, steps, warmup):
+    """12 three-component Float32 fields of uneven cube edges; loss terms and
+    gradient of all of them in one hipGraph replay."""
+    from importlib import import_module
+    B = import_module(pf.__name__ + ".batched")
+    edges = B.draw_edges(12, seed=42)
+    m = B.GradientEnergy(edges, 0)
+    B.taylor_green_fields(m, seed=0)
+    m.capture()
+    for _ in range(warmup):
+        m.replay()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        m.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / steps
+    e0.record(stream)
+    for _ in range(steps):
+        m.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_eager = e0.elapsed_time(e1) / steps
+    nbytes = m.algorithmic_bytes()
+    out = {"workload": "12 Float32 3-component fields, edges %s: loss + gradient" % edges,
+           "ms_per_graph_replay": round(ms_graph, 4), "ms_eager": round(ms_eager, 4),
+           "plan_bytes_per_replay": nbytes,
+           "plan_hbm_frac": round(nbytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "loss": float(m.loss())}
+    del m
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_extras(torch, pf, steps, warmup):
+    res = {}
+    for key, fn in (("plain", extra_plain), ("precision_1024", extra_precision),
+                    ("batched_graph", extra_batched)):
+        try:
+            r = fn(torch, pf, steps, warmup)
+        except Exception as e:     # a failed extra is reported, the headline stands
+            r = {"error": "%s: %s" % (type(e).__name__, e)}
+        if key == "plain":
+            res.update(r)
+        else:
+            res[key] = r
+        print("extra %s done" % key, file=sys.stderr, flush=True)
+    return res
+
+
+# ------------------------------------------------------------------- main
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -99,61 +333,18 @@ def main():
     u = pf.allocate_input(plan)
     v = pf.allocate_output(plan)
     g = torch.Generator(device="cuda").manual_seed(42)
-    if args.transform == "RFFT":
-        u.parent.uniform_(-1, 1, generator=g)
-    else:
-        u.parent.copy_(torch.complex(torch.randn(u.parent.shape, generator=g, device="cuda",
-                                                 dtype=torch.float32 if real_type == T.Float32
-                                                 else torch.float64),
-                                     torch.randn(u.parent.shape, generator=g, device="cuda",
-                                                 dtype=torch.float32 if real_type == T.Float32
-                                                 else torch.float64)))
+    fill(torch, u, g)
 
-    def pair():
-        pf.mul_(v, plan, u)
-        pf.ldiv_(u, plan, v)
-
-    for _ in range(args.warmup):
-        pair()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        pair()
-    e1.record(stream)
-    torch.cuda.synchronize()
+    ms = time_pairs(torch, pf, plan, u, v, args.steps, args.warmup)
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
 
     # per-pass kernel durations from HIP events recorded by the library on the
     # launch stream (pfft_plan_set_timing), averaged over a few pairs
-    plan.set_timing(True)
-    nrep = max(3, min(args.steps, 20))
-    fwd_ms = [0.0] * 3
-    bwd_ms = [0.0] * 3
-    for _ in range(nrep):
-        pf.mul_(v, plan, u)
-        for i, t in enumerate(plan.stage_times()):
-            fwd_ms[i] += t / nrep
-        pf.ldiv_(u, plan, v)
-        for i, t in enumerate(plan.stage_times()):
-            bwd_ms[i] += t / nrep
-    plan.set_timing(False)
-
-    rb = 4 if real_type == T.Float32 else 8
-    X, Y, Z = dims
-    if args.transform == "RFFT":
-        R = X * Y * Z * rb
-        Cb = (X // 2 + 1) * Y * Z * 2 * rb
-        fwd_bytes = [R + Cb, 2 * Cb, 2 * Cb]          # pass 1 (r2c), 2, 3
-        bwd_bytes = [2 * Cb, 2 * Cb, Cb + R]          # pass 3, 2, 1 (c2r) in launch order
-    else:
-        Cb = X * Y * Z * 2 * rb
-        fwd_bytes = [2 * Cb] * 3
-        bwd_bytes = [2 * Cb] * 3
+    fwd_ms, bwd_ms = pass_times(pf, plan, u, v, max(3, min(args.steps, 20)))
+    fwd_bytes = pass_bytes(pf, plan)
+    bwd_bytes = list(reversed(fwd_bytes))
+    X = dims[0]
     pair_bytes = sum(fwd_bytes) + sum(bwd_bytes)
     launches = [(fwd_ms[i], fwd_bytes[i], "fwd%d" % (i + 1)) for i in range(3)]
     launches += [(bwd_ms[i], bwd_bytes[i], "bwd%d" % (3 - i)) for i in range(3)]
@@ -175,7 +366,7 @@ def main():
                 "passes_ms": {name: round(t, 4) for t, _, name in launches},
                 "passes_kernels": kernels}
     out = {
-        "metric": "512³ r2c+c2r FFT: ms/pair, % HBM peak; split-FP16 MFMA vs FP32 at 1e-6 rel",
+        "metric": METRIC,
         "value": round(ms, 4), "unit": "ms/pair", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None,
@@ -193,6 +384,17 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": roofline,
     }
+    del plan, u, v
+    torch.cuda.empty_cache()
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if not args.no_extras:
+        out["extras"] = run_extras(torch, pf, max(5, args.steps // 5), max(2, args.warmup // 5))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dims, args.transform, args.real, args.cpu_reps)
     else:

@@ -461,20 +461,26 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 
 // Lines of a tile: T; lines in flight per pass over the workgroup: TQ =
 // T / LPW (one per L lanes), so each line slot handles LPW lines in turn.
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST>
+// A transposed store goes out in NSP row chunks of M / NSP rows, so the
+// tile needs 1 / NSP of the rows in LDS and more workgroups fit on a CU.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP>
 constexpr size_t lds_bytes() {
   constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
-  constexpr int rows = (KIND == K_R2C || KIND == K_C2R) ? M + 1 : M;
-  constexpr size_t tile = (LD == IO_TILE || ST == IO_TILE) ? (size_t)rows * T * sizeof(C<R>) : 0;
+  constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
+  constexpr int rows_st = M / NSP + (KIND == K_R2C ? 1 : 0);
+  constexpr size_t tld = LD == IO_TILE ? (size_t)rows_ld * T * sizeof(C<R>) : 0;
+  constexpr size_t tst = ST == IO_TILE ? (size_t)rows_st * T * sizeof(C<R>) : 0;
+  constexpr size_t tile = tld > tst ? tld : tst;
   return lines > tile ? lines : tile;
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP>
 __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M>::L, P = Cfg<M>::P;
   constexpr int TQ = T / LPW;
   constexpr int NTHR = TQ * L;
   static_assert(TQ * LPW == T, "LPW must divide T");
+  static_assert(P % NSP == 0 && (NSP == 1 || ST == IO_TILE), "store chunks split P");
   // inputs / outputs per line (the r2c output and c2r input carry M + 1)
   constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
   constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
@@ -646,17 +652,26 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 
     // ---------------- transposed store ----------------
     if constexpr (ST == IO_TILE) {
-      __syncthreads();   // every line slot is read; the tile may overwrite them
+      // chunk h holds rows k = l + L p with p in [h PCH, (h+1) PCH), i.e.
+      // k in [h RCH, (h+1) RCH); the r2c row M rides with the last chunk
+      constexpr int RCH = M / NSP, PCH = P / NSP;
 #pragma unroll
-      for (int q = 0; q < LPW; ++q) {
-        const int t = q * TQ + g;
+      for (int h = 0; h < NSP; ++h) {
+        __syncthreads();   // line slots / the previous chunk are read
 #pragma unroll
-        for (int p = 0; p < P; ++p) tile[tile_idx<R, T>(l + L * p, t)] = v[q][p];
-        if constexpr (KIND == K_R2C)
-          if (l == 0) tile[tile_idx<R, T>(M, t)] = v[q][P];
+        for (int q = 0; q < LPW; ++q) {
+          const int t = q * TQ + g;
+#pragma unroll
+          for (int p = h * PCH; p < (h + 1) * PCH; ++p)
+            tile[tile_idx<R, T>(l + L * p - h * RCH, t)] = v[q][p];
+          if constexpr (KIND == K_R2C)
+            if (h == NSP - 1 && l == 0) tile[tile_idx<R, T>(M - h * RCH, t)] = v[q][P];
+        }
+        __syncthreads();
+        const int64_t ob = c.out_base + (int64_t)(h * RCH) * a.sk_out;
+        if (h == NSP - 1) tile_store<R, T, ROWS_OUT - (NSP - 1) * RCH, NTHR>(tile, a, ob, tid);
+        else tile_store<R, T, RCH, NTHR>(tile, a, ob, tid);
       }
-      __syncthreads();
-      tile_store<R, T, ROWS_OUT, NTHR>(tile, a, c.out_base, tid);
     }
     __syncthreads();
   }
@@ -664,18 +679,18 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads, mf;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP>), g, b, lds, s, a);
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -686,10 +701,11 @@ static void add(std::vector<Entry>& v) {
   e.T = T;
   e.lpw = LPW;
   e.mf = MF;
+  e.nsp = NSP;
   e.threads = T / LPW * Cfg<M>::L;
-  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF>;
+  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP>;
   v.push_back(e);
 }
 
@@ -738,6 +754,14 @@ static void add_size(std::vector<Entry>& v) {
     add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE>(v);
     add<R, M, TT, 2, K_C2R, IO_TILE, IO_LINE>(v);
     add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE>(v);
+    if constexpr (M >= 256 && (M & (M - 1)) == 0) {
+      // row-chunked transposed stores: half (quarter) of the tile in LDS
+      add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+      add<R, M, TT, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+      add<R, M, TT, 2, K_R2C, IO_LINE, IO_TILE, 0, 2>(v);
+      add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE, 0, 2>(v);
+      if constexpr (M == 512) add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE, 0, 4>(v);
+    }
     if constexpr (sizeof(R) == 4) {
       add<R, M, 32, 4, K_C2C, IO_LINE, IO_TILE>(v);
       add<R, M, 32, 4, K_C2C, IO_TILE, IO_LINE>(v);
@@ -749,6 +773,8 @@ static void add_size(std::vector<Entry>& v) {
   } else {
     add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE>(v);
     add<R, M, 8, 1, K_C2C, IO_LINE, IO_TILE>(v);
+    add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+    add<R, M, 8, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
     add<R, M, 8, 2, K_C2C, IO_TILE, IO_LINE>(v);
     add<R, M, 8, 1, K_C2C, IO_TILE, IO_LINE>(v);
     add<R, M, 8, 2, K_R2C, IO_LINE, IO_TILE>(v);
@@ -902,7 +928,8 @@ const char* fast_name(int variant) {
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
-      std::to_string(e.lpw) + (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
+      std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
+      (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();
 }

@@ -68,9 +68,12 @@ static int set_lds_limit(const void* fn, int64_t bytes) {
 static std::mutex g_tune_mu;
 static std::map<std::string, int> g_tune_cache;
 
-static std::string pass_key(const Plan& p, const PassPlan& pp) {
+// The key names the pass signature and the candidate set, so plans whose
+// flags admit different kernels (precision variants) never share a choice.
+static std::string pass_key(const Plan& p, const PassPlan& pp, const std::vector<int>& cands) {
   const PassArgs& a = pp.args;
   std::ostringstream o;
+  for (int v : cands) o << v << ',';
   o << p.real_type << ':' << a.kind << ':' << a.M << ':' << a.n << ':' << a.L_in << ':' << a.L_out
     << ':' << a.sk_in << ':' << a.sk_out << ':' << a.in_cplx << a.out_cplx << ':' << a.D;
   for (int d = 0; d < a.D; ++d) o << ':' << a.ldim[d] << ',' << a.lsin[d] << ',' << a.lsout[d];
@@ -115,7 +118,7 @@ static int autotune(Plan& p) {
       for (PassPlan& pp : *list) {
         std::vector<int> c = fast_candidates(p, pp);
         if (c.empty()) continue;
-        const std::string key = pass_key(p, pp);
+        const std::string key = pass_key(p, pp, c);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
           fast_apply(pp, it->second);

@@ -269,7 +269,9 @@ class PencilFFTPlan:
                                       "arrays must be contiguous GPU tensors")
 
     def _execute(self, direction, src, dst, stream=None):
-        if direction == L.FORWARD:
+        # mul! and the transpose of bmul! read the input pencil; bmul!, ldiv!
+        # and the transpose of mul! read the output pencil
+        if direction in (L.FORWARD, L.ADJOINT_BACKWARD):
             self._check_arrays(src, dst, self.pencil_input, self.pencil_output,
                                self.eltype_input, self.eltype_output)
         else:

@@ -36,7 +36,7 @@ def _case(pf, dims, flag, permute=True):
 
 @pytest.mark.parametrize("flag", ["split16", "bf16x3"])
 @pytest.mark.parametrize("dims,permute", [((1024, 32, 16), True), ((16, 1024, 8), True),
-                                          ((32, 1024, 16), False), ((1024, 1024), True)])
+                                          ((1024, 24, 16), False), ((1024, 1024), True)])
 def test_mfma_variants(pf, gpu, dims, flag, permute):
     ks, err_f, err_b = _case(pf, dims, flag, permute)
     print(dims, flag, ks, "fwd %.2e rt %.2e" % (err_f, err_b))
