@@ -82,6 +82,10 @@ enum {
   PFFT_FLAG_MFMA_SPLIT16 = 1u << 2, /* radix-16 stages on MFMA, split-FP16 */
   PFFT_FLAG_MFMA_BF16X3 = 1u << 3   /* radix-16 stages on MFMA, BF16x3 */
 };
+/* Instead of autotuning, run candidate k (0-based, taken modulo the number
+ * of candidates) of every pass that has specialised kernels: lets a test
+ * exercise each kernel variant the autotuner may pick.  Bits 16..23. */
+#define PFFT_FLAG_CANDIDATE(k) ((uint32_t)(((k) + 1) & 0xff) << 16)
 
 typedef struct pfft_plan pfft_plan;
 

@@ -15,6 +15,11 @@ F32, F64, C32, C64, NOTHING = 0, 1, 2, 3, -1
 FORWARD, BACKWARD, BACKWARD_SCALED, ADJOINT_FORWARD, ADJOINT_BACKWARD = 0, 1, 2, 3, 4
 FLAG_GENERIC_ONLY, FLAG_NO_AUTOTUNE, FLAG_MFMA_SPLIT16, FLAG_MFMA_BF16X3 = 1, 2, 4, 8
 
+
+def FLAG_CANDIDATE(k):
+    """PFFT_FLAG_CANDIDATE(k): run candidate k of every specialised pass."""
+    return ((int(k) + 1) & 0xff) << 16
+
 # every symbol include/pfft.h declares, with (restype, argtypes)
 i32, i64, u32, sz, dbl, vp, cp = (C.c_int32, C.c_int64, C.c_uint32, C.c_size_t, C.c_double,
                                   C.c_void_p, C.c_char_p)

@@ -43,6 +43,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Orders the LDS accesses of one line: a wavefront fence when the line's L
+// lanes are one wavefront, a workgroup barrier when they span several
+// (every thread of the workgroup reaches every line_sync).
+template <int L>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (L <= 64) wave_sync();
+  else __syncthreads();
+}
+
 // ---- forward small DFTs in registers ----------------------------------------
 template <typename R>
 __device__ __forceinline__ void dft16(C<R>* v) {
@@ -123,6 +132,9 @@ __device__ __forceinline__ void stage(C<R>* v, C<R>* line, int l, const C<R>* __
     }
     dft<R, RAD>(w);
   }
+  // several wavefronts share the line: all of them have read it before any
+  // overwrites it
+  if constexpr (L > 64 && !FROM_REG && !TO_REG) __syncthreads();
 #pragma unroll
   for (int i = 0; i < BPL; ++i) {
     if constexpr (TO_REG) {
@@ -142,13 +154,13 @@ __device__ __forceinline__ void fft_rec(C<R>* v, C<R>* line, int l, const C<R>* 
   constexpr bool LAST = sizeof...(REST) == 0;
   stage<R, M, L, P, RAD, NS, FIRST, LAST>(v, line, l, tw);
   if constexpr (!LAST) {
-    wave_sync();
+    line_sync<L>();
     fft_rec<R, M, L, P, NS * RAD, false, REST...>(v, line, l, tw);
   }
 }
 
-// radix schedules: M = L * P, every radix divides P
-template <int M> struct Cfg;
+// radix schedules: M = L * P, every radix divides P; W wavefronts per line
+template <int M, int W = 1> struct Cfg;
 template <> struct Cfg<64> {
   static constexpr int L = 8, P = 8;
   template <typename R>
@@ -184,22 +196,32 @@ template <> struct Cfg<1024> {
     fft_rec<R, 1024, L, P, 1, true, 16, 16, 4>(v, line, l, tw);
   }
 };
+// two wavefronts per line: half the registers per lane, stages synchronised
+// by workgroup barriers
+template <> struct Cfg<1024, 2> {
+  static constexpr int L = 128, P = 8;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 1024, L, P, 1, true, 8, 8, 4, 4>(v, line, l, tw);
+  }
+};
 
 // ---- radix-16 stages on MFMA (1024-point lines) ------------------------------------
 // A radix-16 butterfly is the 16x16 DFT matrix times the twiddled inputs; in
 // real form a 32x32 matrix G times [Re x; Im x].  One wavefront owns a line's
-// 64 butterflies as two 32-column blocks of v_mfma_f32_32x32x8 (four K-chunks
-// of 8).  Lane (j = l % 32, h = l / 32) holds points r in
-// {4h..4h+3, 8+4h..11+4h} of butterfly j -- both the B-operand layout of the
-// instruction and the layout of its accumulators, so inputs and outputs of a
-// lane are the same eight points.
+// 64 butterflies as two 32-column blocks of v_mfma_f32_32x32x16 (gfx950; two
+// K-chunks of 16: the real parts, then the imaginary parts).  Lane (j = l % 32,
+// h = l / 32) feeds points 8h..8h+7 of butterfly j (its B-operand slice) and
+// receives points {4h..4h+3, 8+4h..11+4h} (its accumulator rows); the stage
+// goes LDS to LDS, so the two sets need not coincide.
 //   MF = 1  split-FP16: x = x_hi + x_lo and G = G_hi + G_lo in FP16,
 //           G_hi x_hi + G_hi x_lo + G_lo x_hi accumulated in FP32; the line
 //           is scaled by a power of two (exact) so that |x| < 2^14.
 //   MF = 2  BF16x3: three-term BF16 splits, the six products whose order is at
 //           most the third term, accumulated in FP32.
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __constant__ float kCos16[16] = {
@@ -233,16 +255,17 @@ __device__ __forceinline__ float bf2f(short b) {
 }
 
 template <int MF>
-struct MfmaA {   // this lane's A operands, four K-chunks
-  f16x4 hi[4], lo[4];
-  bf16x4 b1[4], b2[4], b3[4];
+struct MfmaA {   // this lane's A operands, two K-chunks
+  f16x8 hi[2], lo[2];
+  bf16x8 b1[2], b2[2], b3[2];
   __device__ void init(int l) {
     const int i = l & 31, h = l >> 5;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 2; ++c) {
+      s16x8 t1, t2, t3;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const float g = g16(i, 8 * c + 4 * h + t);
+      for (int t = 0; t < 8; ++t) {
+        const float g = g16(i, 16 * c + 8 * h + t);
         if constexpr (MF == 1) {
           const _Float16 gh = (_Float16)g;
           hi[c][t] = gh;
@@ -251,11 +274,17 @@ struct MfmaA {   // this lane's A operands, four K-chunks
           const short a1 = f2bf(g);
           const float r1 = g - bf2f(a1);
           const short a2 = f2bf(r1);
-          b1[c][t] = a1;
-          b2[c][t] = a2;
-          b3[c][t] = f2bf(r1 - bf2f(a2));
+          t1[t] = a1;
+          t2[t] = a2;
+          t3[t] = f2bf(r1 - bf2f(a2));
         }
       }
+      if constexpr (MF == 2) {
+        b1[c] = __builtin_bit_cast(bf16x8, t1);
+        b2[c] = __builtin_bit_cast(bf16x8, t2);
+        b3[c] = __builtin_bit_cast(bf16x8, t3);
+      }
+    }
   }
 };
 
@@ -265,103 +294,116 @@ __device__ __forceinline__ float wave_max(float x) {
   return x;
 }
 
+// G [Re x; Im x] for one 32-butterfly block; x[t] is point 8h + t of this
+// lane's butterfly.  K-chunk c = 0 holds the real parts, c = 1 the imaginary
+// parts.
 template <int MF>
-__device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float>* x) {
-  // x[t]: the lane's eight points; K-chunk c holds s = 8c + 4h + t, i.e.
-  // c = 0: Re x[0..3], 1: Re x[4..7], 2: Im x[0..3], 3: Im x[4..7]
+__device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float> (&x)[8]) {
   f32x16 d = {};
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    float b[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) b[t] = c < 2 ? x[4 * c + t].re : x[4 * (c - 2) + t].im;
+  for (int c = 0; c < 2; ++c) {
     if constexpr (MF == 1) {
-      f16x4 bh, bl;
+      f16x8 bh, bl;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        bh[t] = (_Float16)b[t];
-        bl[t] = (_Float16)(b[t] - (float)bh[t]);
+      for (int t = 0; t < 8; ++t) {
+        const float b = c == 0 ? x[t].re : x[t].im;
+        bh[t] = (_Float16)b;
+        bl[t] = (_Float16)(b - (float)bh[t]);
       }
-      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.hi[c], bh, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.hi[c], bl, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8f16(A.lo[c], bh, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(A.hi[c], bh, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(A.hi[c], bl, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(A.lo[c], bh, d, 0, 0, 0);
     } else {
-      bf16x4 x1, x2, x3;
+      s16x8 t1, t2, t3;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        x1[t] = f2bf(b[t]);
-        const float r1 = b[t] - bf2f(x1[t]);
-        x2[t] = f2bf(r1);
-        x3[t] = f2bf(r1 - bf2f(x2[t]));
+      for (int t = 0; t < 8; ++t) {
+        const float b = c == 0 ? x[t].re : x[t].im;
+        t1[t] = f2bf(b);
+        const float r1 = b - bf2f(t1[t]);
+        t2[t] = f2bf(r1);
+        t3[t] = f2bf(r1 - bf2f(t2[t]));
       }
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x1, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x2, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b2[c], x1, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b1[c], x3, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b3[c], x1, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A.b2[c], x2, d, 0, 0, 0);
+      const bf16x8 x1 = __builtin_bit_cast(bf16x8, t1);
+      const bf16x8 x2 = __builtin_bit_cast(bf16x8, t2);
+      const bf16x8 x3 = __builtin_bit_cast(bf16x8, t3);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b1[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b1[c], x2, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b2[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b1[c], x3, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b3[c], x1, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.b2[c], x2, d, 0, 0, 0);
     }
   }
   return d;
 }
 
 // One radix-16 Stockham stage of a 1024-point line on MFMA, LDS to LDS.
-template <int MF, int NS>
+// With one wavefront per line (L = 64) the wave multiplies both 32-butterfly
+// blocks; with two (L = 128) wavefront w multiplies block w.
+template <int MF, int NS, int L>
 __device__ __forceinline__ void stage16_mfma(C<float>* line, int l, const C<float>* __restrict__ tw,
                                              const MfmaA<MF>& A) {
   constexpr int M = 1024, NB = M / 16, STEP = M / (NS * 16);
-  const int jj = l & 31, h = l >> 5;
-  C<float> x[2][8];
+  constexpr int NBLK = L == 64 ? 2 : 1;   // blocks per wavefront
+  static_assert(L == 64 || L == 128, "one or two wavefronts per line");
+  const int jj = l & 31, h = (l >> 5) & 1, w = l >> 6;
+  C<float> x[NBLK][8];
 #pragma unroll
-  for (int be = 0; be < 2; ++be) {
-    const int b = 32 * be + jj;
+  for (int k = 0; k < NBLK; ++k) {
+    const int b = 32 * (NBLK == 2 ? k : w) + jj;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int r = t < 4 ? 4 * h + t : 8 + 4 * h + (t - 4);
+      const int r = 8 * h + t;
       C<float> v = line[pad8(b + NB * r)];
       if constexpr (NS > 1) v = v * tw[r * (b % NS) * STEP];
-      x[be][t] = v;
+      x[k][t] = v;
     }
   }
   int e = 0;
   if constexpr (MF == 1) {
     float mx = 0.f;
 #pragma unroll
-    for (int be = 0; be < 2; ++be)
+    for (int k = 0; k < NBLK; ++k)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) mx = fmaxf(mx, fmaxf(fabsf(x[be][t].re), fabsf(x[be][t].im)));
+      for (int t = 0; t < 8; ++t) mx = fmaxf(mx, fmaxf(fabsf(x[k][t].re), fabsf(x[k][t].im)));
     mx = wave_max(mx);
-    e = mx > 0.f ? 13 - ilogbf(mx) : 0;   // |x| 2^e < 2^14
+    e = mx > 0.f ? 13 - ilogbf(mx) : 0;   // |x| 2^e < 2^14 over the wave's blocks
 #pragma unroll
-    for (int be = 0; be < 2; ++be)
+    for (int k = 0; k < NBLK; ++k)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) x[be][t] = mk(ldexpf(x[be][t].re, e), ldexpf(x[be][t].im, e));
+      for (int t = 0; t < 8; ++t) x[k][t] = mk(ldexpf(x[k][t].re, e), ldexpf(x[k][t].im, e));
   }
+  f32x16 d[NBLK];
 #pragma unroll
-  for (int be = 0; be < 2; ++be) {
-    const f32x16 d = mfma_block<MF>(A, x[be]);
-    const int b = 32 * be + jj;
+  for (int k = 0; k < NBLK; ++k) d[k] = mfma_block<MF>(A, x[k]);
+  // in place: one wavefront's LDS reads precede its writes in program order
+  // and complete in order; a second wavefront on the line waits at a barrier
+  if constexpr (L > 64) __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NBLK; ++k) {
+    const int b = 32 * (NBLK == 2 ? k : w) + jj;
     const int base = (b / NS) * NS * 16 + (b % NS);
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int q = t < 4 ? 4 * h + t : 8 + 4 * h + (t - 4);
-      line[pad8(base + q * NS)] = mk(ldexpf(d[t], -e), ldexpf(d[8 + t], -e));
+      line[pad8(base + q * NS)] = mk(ldexpf(d[k][t], -e), ldexpf(d[k][8 + t], -e));
     }
   }
 }
 
-// 1024 = 16 (MFMA) x 16 (MFMA) x 4 (registers)
-template <int MF>
+// 1024 = 16 (MFMA) x 16 (MFMA) x 4 (registers); P = 1024 / L points per lane
+template <int MF, int L>
 __device__ __forceinline__ void fft1024_mfma(C<float>* v, C<float>* line, int l,
                                              const C<float>* tw, const MfmaA<MF>& A) {
+  constexpr int P = 1024 / L;
 #pragma unroll
-  for (int p = 0; p < 16; ++p) line[pad8(l + 64 * p)] = v[p];
-  wave_sync();
-  stage16_mfma<MF, 1>(line, l, tw, A);
-  wave_sync();
-  stage16_mfma<MF, 16>(line, l, tw, A);
-  wave_sync();
-  stage<float, 1024, 64, 16, 4, 256, false, true>(v, line, l, tw);
+  for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[p];
+  line_sync<L>();
+  stage16_mfma<MF, 1, L>(line, l, tw, A);
+  line_sync<L>();
+  stage16_mfma<MF, 16, L>(line, l, tw, A);
+  line_sync<L>();
+  stage<float, 1024, L, P, 4, 256, false, true>(v, line, l, tw);
 }
 
 // mixed radix: P is a multiple of every radix of the schedule
@@ -464,7 +506,7 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 // A transposed store goes out in NSP row chunks of M / NSP rows, so the
 // tile needs 1 / NSP of the rows in LDS and more workgroups fit on a CU.
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP>
-constexpr size_t lds_bytes() {
+constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per line
   constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
   constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
   constexpr int rows_st = M / NSP + (KIND == K_R2C ? 1 : 0);
@@ -474,9 +516,9 @@ constexpr size_t lds_bytes() {
   return lines > tile ? lines : tile;
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP>
-__global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
-  constexpr int L = Cfg<M>::L, P = Cfg<M>::P;
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W>
+__global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
+  constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
   constexpr int TQ = T / LPW;
   constexpr int NTHR = TQ * L;
   static_assert(TQ * LPW == T, "LPW must divide T");
@@ -521,11 +563,11 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
       if constexpr (GATHER) {
         // element x_k of this line, from the tile or from the line's LDS slot
         if constexpr (LD == IO_LINE) {
-          if (q > 0) wave_sync();
+          if (q > 0) line_sync<L>();
           const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
           if (active)
             for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = in[k];
-          wave_sync();
+          line_sync<L>();
         }
         auto X = [&](int k) -> C<R> {
           if constexpr (LD == IO_TILE) return tile[tile_idx<R, T>(k, t)];
@@ -575,24 +617,24 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
       }
     }
     if constexpr (LD == IO_TILE) __syncthreads();   // line slots alias the tile
-    else if constexpr (GATHER) wave_sync();
+    else if constexpr (GATHER) line_sync<L>();
 
     // ---------------- FFT, post-process, line stores ----------------
 #pragma unroll
     for (int q = 0; q < LPW; ++q) {
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
-      if (q > 0) wave_sync();
-      if constexpr (MF != 0) fft1024_mfma<MF>((C<float>*)v[q], (C<float>*)line, l,
-                                              (const C<float>*)tw, mfA);
-      else Cfg<M>::template fft<R>(v[q], line, l, tw);
+      if (q > 0) line_sync<L>();
+      if constexpr (MF != 0) fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l,
+                                                 (const C<float>*)tw, mfA);
+      else Cfg<M, W>::template fft<R>(v[q], line, l, tw);
       if constexpr (POST) {
         // the transform lands in natural order in the line's LDS slot
-        wave_sync();
+        line_sync<L>();
 #pragma unroll
         for (int p = 0; p < P; ++p)
           line[pad8(l + L * p)] = KIND == K_DCT3C ? conj(v[q][p]) : v[q][p];
-        wave_sync();
+        line_sync<L>();
         C<R> Y[NX];
 #pragma unroll
         for (int p = 0; p < NX; ++p) {
@@ -679,18 +721,19 @@ __global__ __launch_bounds__(T / LPW * Cfg<M>::L) void k_fast(PassArgs a) {
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>), g, b, lds, s, a);
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1,
+          int W = 1>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -702,10 +745,11 @@ static void add(std::vector<Entry>& v) {
   e.lpw = LPW;
   e.mf = MF;
   e.nsp = NSP;
-  e.threads = T / LPW * Cfg<M>::L;
+  e.w = W;
+  e.threads = T / LPW * Cfg<M, W>::L;
   e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP>;
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>;
   v.push_back(e);
 }
 
@@ -716,9 +760,15 @@ template <int MF>
 static void add_mfma(std::vector<Entry>& v) {
   add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, MF>(v);
   add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, MF>(v);
-  add<float, 1024, 8, 2, K_C2C, IO_LINE, IO_TILE, MF>(v);
+  add<float, 1024, 16, 2, K_C2C, IO_LINE, IO_TILE, MF, 2>(v);
   add<float, 1024, 8, 1, K_C2C, IO_TILE, IO_LINE, MF>(v);
-  add<float, 1024, 8, 2, K_C2C, IO_TILE, IO_LINE, MF>(v);
+  // two wavefronts per line: one 32-butterfly block per wavefront
+  add<float, 1024, 2, 1, K_C2C, IO_LINE, IO_LINE, MF, 1, 2>(v);
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, MF, 1, 2>(v);
+  add<float, 1024, 16, 4, K_C2C, IO_LINE, IO_TILE, MF, 4, 2>(v);
+  add<float, 1024, 16, 4, K_C2C, IO_LINE, IO_TILE, MF, 2, 2>(v);
+  add<float, 1024, 8, 2, K_C2C, IO_LINE, IO_TILE, MF, 2, 2>(v);
+  add<float, 1024, 8, 2, K_C2C, IO_TILE, IO_LINE, MF, 1, 2>(v);
 }
 
 template <typename R, int M>
@@ -775,6 +825,19 @@ static void add_size(std::vector<Entry>& v) {
     add<R, M, 8, 1, K_C2C, IO_LINE, IO_TILE>(v);
     add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
     add<R, M, 8, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+    // 128-byte runs: a 16-line tile in row chunks
+    add<R, M, 16, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+    add<R, M, 16, 4, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+    add<R, M, 16, 4, K_C2C, IO_LINE, IO_TILE, 0, 4>(v);
+    if constexpr (sizeof(R) == 4) {
+      // two wavefronts per line (8 points per lane)
+      add<R, M, 2, 1, K_C2C, IO_LINE, IO_LINE, 0, 1, 2>(v);
+      add<R, M, 4, 1, K_C2C, IO_LINE, IO_LINE, 0, 1, 2>(v);
+      add<R, M, 16, 4, K_C2C, IO_LINE, IO_TILE, 0, 2, 2>(v);
+      add<R, M, 16, 4, K_C2C, IO_LINE, IO_TILE, 0, 4, 2>(v);
+      add<R, M, 8, 2, K_C2C, IO_LINE, IO_TILE, 0, 2, 2>(v);
+      add<R, M, 8, 2, K_C2C, IO_TILE, IO_LINE, 0, 1, 2>(v);
+    }
     add<R, M, 8, 2, K_C2C, IO_TILE, IO_LINE>(v);
     add<R, M, 8, 1, K_C2C, IO_TILE, IO_LINE>(v);
     add<R, M, 8, 2, K_R2C, IO_LINE, IO_TILE>(v);
@@ -929,6 +992,7 @@ const char* fast_name(int variant) {
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
       std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
+      (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

@@ -118,6 +118,11 @@ static int autotune(Plan& p) {
       for (PassPlan& pp : *list) {
         std::vector<int> c = fast_candidates(p, pp);
         if (c.empty()) continue;
+        const uint32_t forced = (p.flags >> 16) & 0xffu;
+        if (forced) {
+          fast_apply(pp, c[(forced - 1) % c.size()]);
+          continue;
+        }
         const std::string key = pass_key(p, pp, c);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
