@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--real", default="Float32", choices=["Float32", "Float64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--extra", action="append", default=None,
+                    choices=["plain", "precision_1024", "batched_graph"],
+                    help="run only the named extra workload(s)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -297,10 +300,12 @@ def extra_batched(torch, pf, steps, warmup):
     return out
 
 
-def run_extras(torch, pf, steps, warmup):
+def run_extras(torch, pf, steps, warmup, only=None):
     res = {}
     for key, fn in (("plain", extra_plain), ("precision_1024", extra_precision),
                     ("batched_graph", extra_batched)):
+        if only and key not in only:
+            continue
         try:
             r = fn(torch, pf, steps, warmup)
         except Exception as e:     # a failed extra is reported, the headline stands
@@ -394,7 +399,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if not args.no_extras:
-        out["extras"] = run_extras(torch, pf, max(5, args.steps // 5), max(2, args.warmup // 5))
+        out["extras"] = run_extras(torch, pf, max(5, args.steps // 5), max(2, args.warmup // 5),
+                                   args.extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dims, args.transform, args.real, args.cpu_reps)
     else:

@@ -23,8 +23,11 @@ adjoint chain:
     uh_bar  = sum_j conj(i k_j) gh_bar_j
     u_bar   = (mul!)^T uh_bar            PFFT_ADJOINT_FORWARD
 
-No reduction runs inside the captured graph: the loss sum is taken after a
-replay.
+The constant factors of the chain (i, 2, 1/N, -i) are folded into two sets
+of complex multipliers, c_j = i k_j for the derivative and
+cb_j = -2 i k_j / N for the gradient, so each elementwise step is one
+torch kernel.  No reduction runs inside the captured graph: the loss sum is
+taken after a replay.
 """
 import math
 
@@ -57,12 +60,15 @@ class _Group:
         kx = torch.arange(X, dtype=_REAL[real_type], device=device)
         kf = torch.fft.fftfreq(edge, 1.0 / edge, dtype=_REAL[real_type], device=device)
         self.k = [kx.view(X, 1, 1), kf.view(1, edge, 1), kf.view(1, 1, edge)]
+        ct = _CPLX[real_type]
+        N = self.plan_g.scale_factor
+        self.c = [(1j * k).to(ct) for k in self.k]                   # d/dx_j
+        self.cb = [(-2j / N * k).to(ct) for k in self.k]            # gradient chain
         # static buffers (the graph's inputs and outputs)
         self.u = PencilArray.empty(self.plan_u.pencil_input, self.plan_u.eltype_input, device)
         self.uh = PencilArray.empty(self.plan_u.pencil_output, self.plan_u.eltype_output, device)
         self.gh = PencilArray.empty(self.plan_g.pencil_output, self.plan_g.eltype_output, device)
         self.g = PencilArray.empty(self.plan_g.pencil_input, self.plan_g.eltype_input, device)
-        self.gb = PencilArray.empty(self.plan_g.pencil_input, self.plan_g.eltype_input, device)
         self.ghb = PencilArray.empty(self.plan_g.pencil_output, self.plan_g.eltype_output,
                                      device)
         self.uhb = PencilArray.empty(self.plan_u.pencil_output, self.plan_u.eltype_output,
@@ -76,17 +82,14 @@ class _Group:
         uh = self.uh.parent                                             # (f, c, x', y, z)
         gh = self.gh.parent                                             # (f, c, j, x', y, z)
         for j in range(3):
-            torch.mul(uh, self.k[j], out=gh[:, :, j])
-            gh[:, :, j].mul_(1j)                                        # i k_j u^
+            torch.mul(uh, self.c[j], out=gh[:, :, j])                   # i k_j u^
         pg._execute(L.BACKWARD_SCALED, self.gh, self.g)                 # g = ldiv(i k u^)
-        torch.mul(self.g.parent, 2.0, out=self.gb.parent)               # dL/dg
-        pg._execute(L.ADJOINT_BACKWARD, self.gb, self.ghb)              # (bmul!)^T
+        # dL/dg = 2 g; (ldiv!)^T = (bmul!)^T / N; the 2 / N ride in cb_j
+        pg._execute(L.ADJOINT_BACKWARD, self.g, self.ghb)
         ghb = self.ghb.parent
-        ghb.mul_(1.0 / pg.scale_factor)                                 # ... of ldiv!
         uhb = self.uhb.parent
-        torch.mul(ghb[:, :, 0], self.k[0], out=uhb)
-        uhb.addcmul_(ghb[:, :, 1], self.k[1]).addcmul_(ghb[:, :, 2], self.k[2])
-        uhb.mul_(-1j)                                                   # sum_j conj(i k_j) .
+        torch.mul(ghb[:, :, 0], self.cb[0], out=uhb)                    # sum_j conj(i k_j) .
+        uhb.addcmul_(ghb[:, :, 1], self.cb[1]).addcmul_(ghb[:, :, 2], self.cb[2])
         pu._execute(L.ADJOINT_FORWARD, self.uhb, self.ub)               # (mul!)^T
 
     def loss(self):

@@ -34,6 +34,9 @@ using C = cpx<R>;
 
 // one padding complex every 8 spreads the radix-8 scatter over the banks
 __device__ __forceinline__ constexpr int pad8(int q) { return q + (q >> 3); }
+// pad8(q + off) for a compile-time offset that is a multiple of 8: the pad
+// is additive, so a lane keeps one base address pad8(q) for all its accesses
+__device__ __forceinline__ constexpr int pad8_off(int off) { return off + (off >> 3); }
 template <int M>
 constexpr int line_pitch() { return M + (M >> 3) + 2; }  // holds indices 0..M
 
@@ -110,11 +113,15 @@ __device__ __forceinline__ void stage(C<R>* v, C<R>* line, int l, const C<R>* __
   static_assert(BPL * RAD == P, "radix must divide the points per lane");
   static_assert(!TO_REG || NS * RAD == M, "registers hold the output of the last stage only");
   C<R> u[P];
+  const C<R>* lp = line + pad8(l);
 #pragma unroll
   for (int i = 0; i < BPL; ++i)
 #pragma unroll
-    for (int r = 0; r < RAD; ++r)
-      u[i * RAD + r] = FROM_REG ? v[i + BPL * r] : line[pad8(l + L * i + NB * r)];
+    for (int r = 0; r < RAD; ++r) {
+      if constexpr (FROM_REG) u[i * RAD + r] = v[i + BPL * r];
+      else if constexpr (L % 8 == 0 && NB % 8 == 0) u[i * RAD + r] = lp[pad8_off(L * i + NB * r)];
+      else u[i * RAD + r] = line[pad8(l + L * i + NB * r)];
+    }
 #pragma unroll
   for (int i = 0; i < BPL; ++i) {
     C<R>* w = u + i * RAD;
@@ -215,10 +222,18 @@ template <> struct Cfg<1024, 2> {
 // receives points {4h..4h+3, 8+4h..11+4h} (its accumulator rows); the stage
 // goes LDS to LDS, so the two sets need not coincide.
 //   MF = 1  split-FP16: x = x_hi + x_lo and G = G_hi + G_lo in FP16,
-//           G_hi x_hi + G_hi x_lo + G_lo x_hi accumulated in FP32; the line
-//           is scaled by a power of two (exact) so that |x| < 2^14.
+//           G_hi x_hi + G_hi x_lo + G_lo x_hi accumulated in FP32.  The line
+//           is scaled once, on entry, by a power of two (exact) so that
+//           |x| < 2^10: the first stage's outputs stay below 2^14 and both
+//           stages' FP16 operands in range; the scale leaves with the
+//           output scaling of the pass.
 //   MF = 2  BF16x3: three-term BF16 splits, the six products whose order is at
-//           most the third term, accumulated in FP32.
+//           most the third term, accumulated in FP32 (BF16 has the FP32
+//           exponent range: no scaling).
+// The data splits round the magnitude to the top 11 (FP16) or 8 (BF16)
+// significant bits with integer arithmetic on the bit pattern (nearest,
+// ties away), so hi converts exactly and the remainder x - hi is exact in
+// FP32.
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
@@ -307,8 +322,9 @@ __device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float> 
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const float b = c == 0 ? x[t].re : x[t].im;
-        bh[t] = (_Float16)b;
-        bl[t] = (_Float16)(b - (float)bh[t]);
+        const float hf = __uint_as_float((__float_as_uint(b) + 0x1000u) & 0xffffe000u);
+        bh[t] = (_Float16)hf;
+        bl[t] = (_Float16)(b - hf);
       }
       d = __builtin_amdgcn_mfma_f32_32x32x16_f16(A.hi[c], bh, d, 0, 0, 0);
       d = __builtin_amdgcn_mfma_f32_32x32x16_f16(A.hi[c], bl, d, 0, 0, 0);
@@ -318,10 +334,13 @@ __device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float> 
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const float b = c == 0 ? x[t].re : x[t].im;
-        t1[t] = f2bf(b);
-        const float r1 = b - bf2f(t1[t]);
-        t2[t] = f2bf(r1);
-        t3[t] = f2bf(r1 - bf2f(t2[t]));
+        const unsigned u1 = (__float_as_uint(b) + 0x8000u) & 0xffff0000u;
+        const float r1 = b - __uint_as_float(u1);
+        const unsigned u2 = (__float_as_uint(r1) + 0x8000u) & 0xffff0000u;
+        const float r2 = r1 - __uint_as_float(u2);
+        t1[t] = (short)(u1 >> 16);
+        t2[t] = (short)(u2 >> 16);
+        t3[t] = (short)((__float_as_uint(r2) + 0x8000u) >> 16);
       }
       const bf16x8 x1 = __builtin_bit_cast(bf16x8, t1);
       const bf16x8 x2 = __builtin_bit_cast(bf16x8, t2);
@@ -340,70 +359,87 @@ __device__ __forceinline__ f32x16 mfma_block(const MfmaA<MF>& A, const C<float> 
 // One radix-16 Stockham stage of a 1024-point line on MFMA, LDS to LDS.
 // With one wavefront per line (L = 64) the wave multiplies both 32-butterfly
 // blocks; with two (L = 128) wavefront w multiplies block w.
+// Addresses in closed form (butterfly b = 32 blk + jj, point r = 8h + t):
+//   read   pad8(b + 64 r)            = pad8(jj) + 576 h + 36 blk + 72 t
+//   write  NS = 1:  pad8(16 b + q)   = 18 jj + 4 h + 576 blk + 9 g + t'
+//          NS = 16: pad8(256 (b / 16) + b % 16 + 16 q)
+//                   = 288 (jj / 16) + pad8(jj % 16) + 72 h + 576 blk + 144 g + 18 t'
+// with output point q = 4h + t' + 8g (t' = t % 4, g = t / 4), so each lane
+// keeps one base address and every access takes a constant offset.
 template <int MF, int NS, int L>
 __device__ __forceinline__ void stage16_mfma(C<float>* line, int l, const C<float>* __restrict__ tw,
                                              const MfmaA<MF>& A) {
-  constexpr int M = 1024, NB = M / 16, STEP = M / (NS * 16);
-  constexpr int NBLK = L == 64 ? 2 : 1;   // blocks per wavefront
+  static_assert(NS == 1 || NS == 16, "the two radix-16 stages of 1024 = 16 x 16 x 4");
   static_assert(L == 64 || L == 128, "one or two wavefronts per line");
-  const int jj = l & 31, h = (l >> 5) & 1, w = l >> 6;
+  constexpr int NBLK = L == 64 ? 2 : 1;   // blocks per wavefront
+  const int jj = l & 31, h = (l >> 5) & 1, w = L == 64 ? 0 : (l >> 6);
+  const C<float>* rd = line + pad8(jj) + 576 * h + 36 * w;
   C<float> x[NBLK][8];
 #pragma unroll
-  for (int k = 0; k < NBLK; ++k) {
-    const int b = 32 * (NBLK == 2 ? k : w) + jj;
+  for (int k = 0; k < NBLK; ++k)
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int r = 8 * h + t;
-      C<float> v = line[pad8(b + NB * r)];
-      if constexpr (NS > 1) v = v * tw[r * (b % NS) * STEP];
+      C<float> v = rd[36 * k + 72 * t];
+      if constexpr (NS > 1) v = v * tw[(8 * h + t) * (jj & 15) * 4];
       x[k][t] = v;
     }
-  }
-  int e = 0;
-  if constexpr (MF == 1) {
-    float mx = 0.f;
-#pragma unroll
-    for (int k = 0; k < NBLK; ++k)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) mx = fmaxf(mx, fmaxf(fabsf(x[k][t].re), fabsf(x[k][t].im)));
-    mx = wave_max(mx);
-    e = mx > 0.f ? 13 - ilogbf(mx) : 0;   // |x| 2^e < 2^14 over the wave's blocks
-#pragma unroll
-    for (int k = 0; k < NBLK; ++k)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) x[k][t] = mk(ldexpf(x[k][t].re, e), ldexpf(x[k][t].im, e));
-  }
   f32x16 d[NBLK];
 #pragma unroll
   for (int k = 0; k < NBLK; ++k) d[k] = mfma_block<MF>(A, x[k]);
   // in place: one wavefront's LDS reads precede its writes in program order
   // and complete in order; a second wavefront on the line waits at a barrier
   if constexpr (L > 64) __syncthreads();
+  C<float>* wr = NS == 1 ? line + 18 * jj + 4 * h + 576 * w
+                         : line + 288 * (jj >> 4) + pad8(jj & 15) + 72 * h + 576 * w;
 #pragma unroll
-  for (int k = 0; k < NBLK; ++k) {
-    const int b = 32 * (NBLK == 2 ? k : w) + jj;
-    const int base = (b / NS) * NS * 16 + (b % NS);
+  for (int k = 0; k < NBLK; ++k)
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int q = t < 4 ? 4 * h + t : 8 + 4 * h + (t - 4);
-      line[pad8(base + q * NS)] = mk(ldexpf(d[k][t], -e), ldexpf(d[k][8 + t], -e));
+      const int off = NS == 1 ? 576 * k + 9 * (t >> 2) + (t & 3)
+                              : 576 * k + 144 * (t >> 2) + 18 * (t & 3);
+      wr[off] = mk(d[k][t], d[k][8 + t]);
     }
-  }
 }
 
-// 1024 = 16 (MFMA) x 16 (MFMA) x 4 (registers); P = 1024 / L points per lane
+// 1024 = 16 (MFMA) x 16 (MFMA) x 4 (registers); P = 1024 / L points per lane.
+// Returns e: the outputs in v are the transform times 2^e.
 template <int MF, int L>
-__device__ __forceinline__ void fft1024_mfma(C<float>* v, C<float>* line, int l,
-                                             const C<float>* tw, const MfmaA<MF>& A) {
+__device__ __forceinline__ int fft1024_mfma(C<float>* v, C<float>* line, int l,
+                                            const C<float>* tw, const MfmaA<MF>& A) {
   constexpr int P = 1024 / L;
+  int e = 0;
+  if constexpr (MF == 1) {
+    // largest magnitude as bits: |x| orders like its bit pattern
+    unsigned mb = 0;
 #pragma unroll
-  for (int p = 0; p < P; ++p) line[pad8(l + L * p)] = v[p];
+    for (int p = 0; p < P; ++p)
+      mb = max(mb, max(__float_as_uint(v[p].re) & 0x7fffffffu,
+                       __float_as_uint(v[p].im) & 0x7fffffffu));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o, 64));
+    if constexpr (L > 64) {
+      // the line's two wavefronts exchange their maxima through the spare
+      // slot at index 1152 (c2c lines use indices up to pad8(1023) = 1150)
+      unsigned* ex = (unsigned*)(line + 1152);
+      if ((l & 63) == 0) ex[l >> 6] = mb;
+      __syncthreads();
+      mb = max(ex[0], ex[1]);
+    }
+    // |x| 2^e < 2^10 (a subnormal maximum counts as exponent -127: still below)
+    e = mb ? 9 - ((int)(mb >> 23) - 127) : 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = mk(ldexpf(v[p].re, e), ldexpf(v[p].im, e));
+  }
+  C<float>* lp = line + pad8(l);
+#pragma unroll
+  for (int p = 0; p < P; ++p) lp[pad8_off(L * p)] = v[p];
   line_sync<L>();
   stage16_mfma<MF, 1, L>(line, l, tw, A);
   line_sync<L>();
   stage16_mfma<MF, 16, L>(line, l, tw, A);
   line_sync<L>();
   stage<float, 1024, L, P, 4, 256, false, true>(v, line, l, tw);
+  return e;
 }
 
 // mixed radix: P is a multiple of every radix of the schedule
@@ -536,6 +572,29 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
   const int l = tid - g * L;     // lane within the line
   C<R>* line = lds + g * line_pitch<M>();
   C<R>* tile = lds;
+  // Lane bases for the points k = l + L p and M - k = (M - L (p+1)) + (L - l):
+  // pad8 is additive over multiples of 8 and the tile swizzle repeats every
+  // SWP rows, so with L a multiple of those each access is base + constant.
+  constexpr int SWP = sizeof(R) == 4 ? T / 2 : T;
+  constexpr bool ADD8 = L % 8 == 0, ADDT = L % SWP == 0;
+  C<R>* const lp = line + pad8(l);
+  C<R>* const lm = line + pad8(L - l);
+  auto LK = [&](int p) -> C<R>& {   // line[pad8(l + L p)]
+    if constexpr (ADD8) return lp[pad8_off(L * p)];
+    else return line[pad8(l + L * p)];
+  };
+  auto LMK = [&](int p) -> C<R>& {  // line[pad8(M - l - L p)], for l + L p > 0
+    if constexpr (ADD8) return lm[pad8_off(M - L * (p + 1))];
+    else return line[pad8(M - l - L * p)];
+  };
+  auto TK = [&](int p, int t, int k0) -> C<R>& {   // tile[tile_idx(k0 + l + L p, t)]
+    if constexpr (ADDT) return tile[tile_idx<R, T>(l, t) + (k0 + L * p) * T];
+    else return tile[tile_idx<R, T>(k0 + l + L * p, t)];
+  };
+  auto TMK = [&](int p, int t) -> C<R>& {          // tile[tile_idx(M - l - L p, t)]
+    if constexpr (ADDT) return tile[tile_idx<R, T>(L - l, t) + (M - L * (p + 1)) * T];
+    else return tile[tile_idx<R, T>(M - l - L * p, t)];
+  };
   const C<R>* __restrict__ tw = (const C<R>*)a.tw;
   const C<R>* __restrict__ aux = (const C<R>*)a.aux;
   // backward FFTs run as conj(FFT(conj x))
@@ -573,13 +632,22 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
           if constexpr (LD == IO_TILE) return tile[tile_idx<R, T>(k, t)];
           else return line[pad8(k)];
         };
+        auto XK = [&](int p) -> C<R> {    // x_k, k = l + L p
+          if constexpr (LD == IO_TILE) return TK(p, t, 0);
+          else return LK(p);
+        };
+        auto XMK = [&](int p) -> C<R> {   // x_{M-k} for k > 0 (x_M when k = 0)
+          if constexpr (LD == IO_TILE) return TMK(p, t);
+          else return LMK(p);
+        };
 #pragma unroll
         for (int p = 0; p < P; ++p) {
           const int k = l + L * p;
           C<R> z;
           if constexpr (KIND == K_C2R) {
             // Hermitian merge: Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w_n^-k
-            C<R> A = X(k), B = conj(X(M - k));
+            // (k = 0 pairs X_0 with X_M, the row M of the input)
+            C<R> A = XK(p), B = conj(XMK(p));
             if (k == 0) { A.im = R(0); B.im = R(0); }
             else if (a.adjoint) { A = scl(A, R(0.5)); B = scl(B, R(0.5)); }  // X_k / w_k
             const C<R> E = A + B;
@@ -590,8 +658,8 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
             z = X(2 * k < M ? 2 * k : 2 * M - 2 * k - 1);
           } else {
             // V_k = w_4M^-k (x_k - i x_{M-k}), x_M = 0
-            const C<R> xk = X(k);
-            const C<R> xr = k == 0 ? mk(R(0), R(0)) : X(M - k);
+            const C<R> xk = XK(p);
+            const C<R> xr = k == 0 ? mk(R(0), R(0)) : XMK(p);
             z = conj(aux[k]) * (xk + mul_mi(xr));
           }
           v[q][p] = conj_in ? conj(z) : z;
@@ -599,7 +667,7 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
       } else if constexpr (LD == IO_TILE) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          const C<R> x = tile[tile_idx<R, T>(l + L * p, t)];
+          const C<R> x = TK(p, t, 0);
           v[q][p] = conj_in ? conj(x) : x;
         }
       } else {
@@ -625,15 +693,26 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
       if (q > 0) line_sync<L>();
-      if constexpr (MF != 0) fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l,
-                                                 (const C<float>*)tw, mfA);
-      else Cfg<M, W>::template fft<R>(v[q], line, l, tw);
+      [[maybe_unused]] R scq = sc;   // output scaling of this line
+      if constexpr (MF != 0) {
+        const int e = fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l,
+                                          (const C<float>*)tw, mfA);
+        // remove the entry scaling 2^e: folded into the output factor when
+        // that stays a normal number (sc >= 2^-60 here), else applied apart
+        if (e >= -60 && e <= 60) {
+          scq = ldexpf(sc, -e);
+        } else {
+#pragma unroll
+          for (int p = 0; p < P; ++p) v[q][p] = mk(ldexpf(v[q][p].re, -e), ldexpf(v[q][p].im, -e));
+        }
+      } else {
+        Cfg<M, W>::template fft<R>(v[q], line, l, tw);
+      }
       if constexpr (POST) {
         // the transform lands in natural order in the line's LDS slot
         line_sync<L>();
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-          line[pad8(l + L * p)] = KIND == K_DCT3C ? conj(v[q][p]) : v[q][p];
+        for (int p = 0; p < P; ++p) LK(p) = KIND == K_DCT3C ? conj(v[q][p]) : v[q][p];
         line_sync<L>();
         C<R> Y[NX];
 #pragma unroll
@@ -641,8 +720,8 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
           const int k = p < P ? l + L * p : M;
           if constexpr (KIND == K_R2C) {
             // X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
-            const C<R> Z = line[pad8(k == M ? 0 : k)];
-            const C<R> Zc = conj(line[pad8(k == 0 ? 0 : M - k)]);
+            const C<R> Z = p < P ? LK(p) : line[0];
+            const C<R> Zc = conj(p < P && k != 0 ? LMK(p) : line[0]);
             const C<R> E = scl(Z + Zc, R(0.5));
             const C<R> O = scl(mul_mi(Z - Zc), R(0.5));
             Y[p] = E + aux[k] * O;
@@ -650,8 +729,8 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
           } else if constexpr (KIND == K_DCT2C) {
             // split V into the transforms of the real and imaginary parts,
             // then y_k = 2 Re(w_4M^k V_k) for each
-            const C<R> Vk = line[pad8(k)];
-            const C<R> Vn = conj(line[pad8(k == 0 ? 0 : M - k)]);
+            const C<R> Vk = LK(p);
+            const C<R> Vn = conj(k != 0 ? LMK(p) : line[0]);
             const C<R> Va = scl(Vk + Vn, R(0.5));
             const C<R> Vb = scl(mul_mi(Vk - Vn), R(0.5));
             const C<R> w = aux[k];
@@ -681,7 +760,7 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
         }
       } else {
 #pragma unroll
-        for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], sc);
+        for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], scq);
         if constexpr (ST == IO_LINE) {
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
@@ -704,8 +783,7 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
         for (int q = 0; q < LPW; ++q) {
           const int t = q * TQ + g;
 #pragma unroll
-          for (int p = h * PCH; p < (h + 1) * PCH; ++p)
-            tile[tile_idx<R, T>(l + L * p - h * RCH, t)] = v[q][p];
+          for (int p = h * PCH; p < (h + 1) * PCH; ++p) TK(p, t, -h * RCH) = v[q][p];
           if constexpr (KIND == K_R2C)
             if (h == NSP - 1 && l == 0) tile[tile_idx<R, T>(M - h * RCH, t)] = v[q][P];
         }
