@@ -124,7 +124,17 @@ int pfft_transform_name(int32_t kind, char* buf, size_t buflen);
  * (plans.jl:397-521), the per-dimension binv and scale factors
  * (plans.jl:523-553) and their product (plans.jl:268).  proc_dims must
  * multiply to one process (one GPU).  Creating a plan touches no GPU: device
- * tables are built on the first execute (or by pfft_plan_prepare).          */
+ * tables are built on the first execute (or by pfft_plan_prepare).
+ *
+ * In-place plans (kinds OR-ed with PFFT_INPLACE) stand in for the
+ * reference's in-place PencilFFTPlan over a ManyPencilArray (allocate.jl:57-83,
+ * operations.jl:175-225): one user buffer of pfft_plan_nbytes' in_bytes
+ * (== out_bytes) holds the input view (the first pencil) and the output view
+ * (the last pencil), both at offset 0, and pfft_execute takes it as both
+ * `in` and `out`.  Accepted chains: FFT!/BFFT!/R2R!/NoTransform!, or RFFT!
+ * followed by FFT!s, whose real input view sits in a parent padded to
+ * 2 (n/2 + 1) along memory axis 0 (ManyPencilArrayRFFT!,
+ * multiarrays_r2c.jl:26-79; see pfft_plan_parent_dims).                     */
 int pfft_plan_create(pfft_plan** out, int32_t ndims, const int64_t* dims_logical,
                      const int32_t* kinds, int32_t real_type, int32_t permute_dims,
                      int32_t n_extra, const int64_t* extra_dims, uint32_t flags);
@@ -135,6 +145,10 @@ int pfft_plan_prepare(pfft_plan* plan, void* hip_stream);
 int pfft_plan_ndims(const pfft_plan* plan, int32_t* ndims, int32_t* n_extra);
 int pfft_plan_size_global(const pfft_plan* plan, int32_t which, int64_t* dims);
 int pfft_plan_mem_dims(const pfft_plan* plan, int32_t which, int64_t* dims);
+/* Memory dims of the parent array of the input (0) / output (1) view: the
+ * memory dims, with memory axis 0 padded to 2 (n/2 + 1) for the real input of
+ * an in-place RFFT! plan (multiarrays_r2c.jl:75-86, _make_real_array).      */
+int pfft_plan_parent_dims(const pfft_plan* plan, int32_t which, int64_t* dims);
 int pfft_plan_permutation(const pfft_plan* plan, int32_t which, int32_t* perm);
 int pfft_plan_eltypes(const pfft_plan* plan, int32_t* eltype_in, int32_t* eltype_out);
 int pfft_plan_is_inplace(const pfft_plan* plan, int32_t* inplace);

@@ -8,10 +8,11 @@ from . import _lib
 from . import transforms as Transforms
 from . import autodiff, batched
 from ._lib import ArgumentError, DeviceError, PfftError, UnsupportedError
-from .plans import (Pencil, PencilArray, PencilFFTPlan, Val, allocate_input, allocate_output,
-                    bmul_, execute_many, ldiv_, mul_, scale_factor)
+from .plans import (ManyPencilArray, Pencil, PencilArray, PencilFFTPlan, Val, allocate_input,
+                    allocate_output, bmul_, execute_many, ldiv_, mul_, scale_factor)
 
-__all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "Pencil", "allocate_input",
+__all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "ManyPencilArray", "Pencil",
+           "allocate_input",
            "allocate_output", "mul_", "ldiv_", "bmul_", "scale_factor", "Val",
            "execute_many", "ArgumentError", "UnsupportedError", "DeviceError", "PfftError",
            "autodiff", "batched"]

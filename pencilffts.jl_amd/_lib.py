@@ -42,6 +42,7 @@ SIGNATURES = {
     "pfft_plan_ndims": (C.c_int, [vp, P(i32), P(i32)]),
     "pfft_plan_size_global": (C.c_int, [vp, i32, P(i64)]),
     "pfft_plan_mem_dims": (C.c_int, [vp, i32, P(i64)]),
+    "pfft_plan_parent_dims": (C.c_int, [vp, i32, P(i64)]),
     "pfft_plan_permutation": (C.c_int, [vp, i32, P(i32)]),
     "pfft_plan_eltypes": (C.c_int, [vp, P(i32), P(i32)]),
     "pfft_plan_is_inplace": (C.c_int, [vp, P(i32)]),

@@ -167,6 +167,17 @@ int pfft_plan_mem_dims(const pfft_plan* plan, int32_t which, int64_t* dims) {
   return PFFT_OK;
 }
 
+// Memory dims of the parent array behind the input (which = 0) or output
+// (1) view: the memory dims, except that the real input of an in-place RFFT!
+// plan lives in a parent padded to 2 (n/2 + 1) along memory axis 0
+// (multiarrays_r2c.jl:75-86).
+int pfft_plan_parent_dims(const pfft_plan* plan, int32_t which, int64_t* dims) {
+  int r = pfft_plan_mem_dims(plan, which, dims);
+  if (r) return r;
+  if (which == 0 && plan->p.pad_in0) dims[0] = plan->p.pad_in0;
+  return PFFT_OK;
+}
+
 // 1-based permutation (memory axis i holds logical dim perm[i]).
 int pfft_plan_permutation(const pfft_plan* plan, int32_t which, int32_t* perm) {
   NEED(plan);

@@ -77,6 +77,8 @@ __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, in
   if ((dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) && !p.adjoint_ok)
     return fail(PFFT_ERR_UNSUPPORTED, "no adjoint for R2R plans");
   const bool fwd_shape = dir == PFFT_FORWARD || dir == PFFT_ADJOINT_BACKWARD;
+  if (p.inplace && in != out)
+    return fail(PFFT_ERR_ARGUMENT, "input and output arrays for in-place plan must be the same");
   // the two work buffers as separate exact-size allocations
   std::vector<char> work0(p.work0_bytes), work1(p.work_bytes - p.work0_bytes);
   const std::vector<PassPlan>& passes = fwd_shape ? p.fwd : p.bwd;

@@ -552,8 +552,13 @@ constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per li
   return lines > tile ? lines : tile;
 }
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W>
-__global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
+// OCC > 0 asks the compiler for at least OCC waves per SIMD (a register
+// budget of 512 / OCC): more waves in flight at the price of possible spills;
+// the autotuner keeps such a variant only where it is faster.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
+          int OCC>
+__global__ __launch_bounds__((T / LPW * Cfg<M, W>::L))
+__attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
   constexpr int TQ = T / LPW;
   constexpr int NTHR = TQ * L;
@@ -799,19 +804,20 @@ __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L)) void k_fast(PassArgs a) {
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w, occ;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W>
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
+          int OCC>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>), g, b, lds, s, a);
 }
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1,
-          int W = 1>
+          int W = 1, int OCC = 0>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -824,10 +830,11 @@ static void add(std::vector<Entry>& v) {
   e.mf = MF;
   e.nsp = NSP;
   e.w = W;
+  e.occ = OCC;
   e.threads = T / LPW * Cfg<M, W>::L;
   e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W>;
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>;
   v.push_back(e);
 }
 
@@ -882,6 +889,12 @@ static void add_size(std::vector<Entry>& v) {
     add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE>(v);
     add<R, M, TT, 2, K_C2R, IO_TILE, IO_LINE>(v);
     add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE>(v);
+    if constexpr (sizeof(R) == 4 && (M == 256 || M == 512)) {
+      // register budgets for more waves in flight on the r2c / c2r of 512 and
+      // 1024 reals (small spills, kept only where the tuner finds them faster)
+      add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE, 0, 1, 1, 6>(v);
+      add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE, 0, 1, 1, 8>(v);
+    }
     if constexpr (M >= 256 && (M & (M - 1)) == 0) {
       // row-chunked transposed stores: half (quarter) of the tile in LDS
       add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
@@ -1071,6 +1084,7 @@ const char* fast_name(int variant) {
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
       std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
+      (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

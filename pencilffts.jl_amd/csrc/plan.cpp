@@ -297,12 +297,15 @@ static int radix_schedule(int64_t M, int32_t* rad) {
 // ---- layouts -----------------------------------------------------------------
 // Strides (in elements) of each logical dim and of the extra dim for a memory
 // permutation `perm` of the N logical dims with sizes `dims`.
+// Strides of the layout `perm` over `dims`; a nonzero pad0 is the padded
+// length of memory axis 0 (the real view of an in-place r2c buffer holds
+// 2 (n/2 + 1) reals per line, multiarrays_r2c.jl:75-86).
 static void layout_strides(int N, const int32_t* perm, const int64_t* dims, int64_t* stride,
-                           int64_t* stride_extra) {
+                           int64_t* stride_extra, int64_t pad0 = 0) {
   int64_t s = 1;
   for (int i = 0; i < N; ++i) {
     stride[perm[i]] = s;
-    s *= dims[perm[i]];
+    s *= (i == 0 && pad0 > dims[perm[i]]) ? pad0 : dims[perm[i]];
   }
   *stride_extra = s;
 }
@@ -325,7 +328,8 @@ static void chain_perm(int N, int n, bool permute, int32_t* perm) {
 // into (perm_out, dims_out).
 static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t elt_out,
                       const int32_t* perm_in, const int64_t* dims_in, const int32_t* perm_out,
-                      const int64_t* dims_out, PassPlan& pp) {
+                      const int64_t* dims_out, PassPlan& pp, int64_t pad_in = 0,
+                      int64_t pad_out = 0) {
   const int N = p.N;
   PassArgs& a = pp.args;
   memset(&a, 0, sizeof(a));
@@ -334,16 +338,13 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
   pp.tw_id = -1;
   pp.aux_id = -1;
   pp.variant = 0;
-  pp.in_elems = p.E;
-  pp.out_elems = p.E;
-  for (int d = 0; d < N; ++d) {
-    pp.in_elems *= dims_in[d];
-    pp.out_elems *= dims_out[d];
-  }
 
   int64_t sin_[kMaxDims], sout[kMaxDims], sin_e, sout_e;
-  layout_strides(N, perm_in, dims_in, sin_, &sin_e);
-  layout_strides(N, perm_out, dims_out, sout, &sout_e);
+  layout_strides(N, perm_in, dims_in, sin_, &sin_e, pad_in);
+  layout_strides(N, perm_out, dims_out, sout, &sout_e, pad_out);
+  // element counts of the arrays the pass touches (padding included)
+  pp.in_elems = p.E * sin_e;
+  pp.out_elems = p.E * sout_e;
 
   a.L_in = dims_in[dim];
   a.L_out = dims_out[dim];
@@ -579,9 +580,27 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
   }
   p.elt_out = Ti;
 
-  if (p.inplace)
-    return fail(PFFT_ERR_UNSUPPORTED,
-                "in-place plans (FFT!, RFFT!, ...) are not provided by this build");
+  // In-place plans (allocate.jl:57-83): chains that keep the element type
+  // (FFT!, BFFT!, R2R!, NoTransform!) over one buffer of the largest pencil,
+  // and RFFT! followed by FFT!s over the padded real layout of
+  // multiarrays_r2c.jl (2 (n/2 + 1) reals per first-axis line).
+  int64_t pad_real = 0;
+  if (p.inplace) {
+    const int32_t b0 = kind_base(kinds[0]);
+    bool ok = true;
+    if (b0 == PFFT_RFFT) {
+      for (int d = 1; d < N; ++d) ok = ok && kind_base(kinds[d]) == PFFT_FFT;
+      pad_real = 2 * (p.size_in[0] / 2 + 1);
+    } else {
+      for (int d = 0; d < N; ++d)
+        ok = ok && kind_base(kinds[d]) != PFFT_RFFT && !is_c2r(kind_base(kinds[d]));
+    }
+    if (!ok)
+      return fail(PFFT_ERR_UNSUPPORTED,
+                  "in-place plans take FFT!/BFFT!/R2R!/NoTransform! chains, or RFFT! "
+                  "followed by FFT!s (allocate.jl:57-83)");
+  }
+  p.pad_in0 = pad_real;
 
   // passes
   p.tables.clear();
@@ -610,7 +629,7 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
       else choose(cur, p.stages[n + 1].perm, s.dims_after, outp);
       PassPlan pp;
       int r = build_pass(p, n, s.kind_fwd, s.elt_in, s.elt_out, cur, s.dims_before, outp,
-                         s.dims_after, pp);
+                         s.dims_after, pp, n == 0 ? pad_real : 0, 0);
       if (r) return r;
       p.fwd.push_back(pp);
       memcpy(cur, outp, sizeof(cur));
@@ -626,32 +645,86 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
       else choose(cur, p.stages[n - 1].perm, s.dims_before, outp);
       PassPlan pp;
       int r = build_pass(p, n, s.kind_bwd, s.elt_out, s.elt_in, cur, s.dims_after, outp,
-                         s.dims_before, pp);
+                         s.dims_before, pp, 0, n == 0 ? pad_real : 0);
       if (r) return r;
       p.bwd.push_back(pp);
       memcpy(cur, outp, sizeof(cur));
     }
   }
-  // buffers: user in -> work0 -> work1 -> work0 ... -> user out
   size_t w[2] = {0, 0};
-  for (auto* list : {&p.fwd, &p.bwd}) {
-    const int np = (int)list->size();
-    for (int i = 0; i < np; ++i) {
-      PassPlan& pp = (*list)[i];
-      pp.src = i == 0 ? 0 : 1 + ((i - 1) & 1);
-      pp.dst = i == np - 1 ? 0 : 1 + (i & 1);
-      if (pp.dst) {
-        const size_t bytes = (size_t)pp.out_elems * eltype_bytes(pp.elt_out);
-        w[pp.dst - 1] = std::max(w[pp.dst - 1], bytes);
+  auto use = [&](PassPlan& pp) {
+    if (pp.dst) {
+      const size_t bytes = (size_t)pp.out_elems * eltype_bytes(pp.elt_out);
+      w[pp.dst - 1] = std::max(w[pp.dst - 1], bytes);
+    }
+  };
+  if (!p.inplace) {
+    // buffers: user in -> work0 -> work1 -> work0 ... -> user out
+    for (auto* list : {&p.fwd, &p.bwd}) {
+      const int np = (int)list->size();
+      for (int i = 0; i < np; ++i) {
+        PassPlan& pp = (*list)[i];
+        pp.src = i == 0 ? 0 : 1 + ((i - 1) & 1);
+        pp.dst = i == np - 1 ? 0 : 1 + (i & 1);
+        use(pp);
       }
     }
+    p.in_bytes = (size_t)p.fwd.front().in_elems * eltype_bytes(p.elt_in);
+    p.out_bytes = (size_t)p.fwd.back().out_elems * eltype_bytes(p.elt_out);
+  } else {
+    // One user buffer U holds every pencil (ManyPencilArray, the largest
+    // pencil's size).  A pass whose input and output lines occupy the same
+    // bytes (contiguous on both sides, same line strides in bytes) may run in
+    // place, each line read whole before it is written; any other pass moves
+    // data between U and the scratch W.  The chain starts and ends in U, so
+    // the moves must be even: an odd count turns one in-place-capable pass
+    // into a move, and failing that the chain uses two scratch buffers.
+    size_t ubytes = 0;
+    for (auto* list : {&p.fwd, &p.bwd})
+      for (const PassPlan& pp : *list) {
+        ubytes = std::max(ubytes, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
+        ubytes = std::max(ubytes, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
+      }
+    for (auto* list : {&p.fwd, &p.bwd}) {
+      const int np = (int)list->size();
+      std::vector<bool> same(np);
+      int moves = 0, first_same = -1;
+      for (int i = 0; i < np; ++i) {
+        const PassArgs& a = (*list)[i].args;
+        const int64_t bi = eltype_bytes((*list)[i].elt_in), bo = eltype_bytes((*list)[i].elt_out);
+        bool ok = a.sk_in == 1 && a.sk_out == 1;
+        for (int d = 0; d < a.D && ok; ++d) ok = a.lsin[d] * bi == a.lsout[d] * bo;
+        same[i] = ok;
+        if (!ok) ++moves;
+        else if (first_same < 0) first_same = i;
+      }
+      if (moves % 2 == 0 || first_same >= 0) {
+        int cur = 0;   // 0 = U, 1 = W
+        for (int i = 0; i < np; ++i) {
+          PassPlan& pp = (*list)[i];
+          const bool move = !same[i] || (moves % 2 == 1 && i == first_same);
+          pp.src = cur;
+          pp.dst = move ? 1 - cur : cur;
+          cur = pp.dst;
+          use(pp);
+        }
+      } else {
+        if (np < 2)
+          return fail(PFFT_ERR_UNSUPPORTED, "in-place plan with a single transposing pass");
+        for (int i = 0; i < np; ++i) {   // U -> W0 -> W1 -> W0 ... -> U
+          PassPlan& pp = (*list)[i];
+          pp.src = i == 0 ? 0 : 1 + ((i - 1) & 1);
+          pp.dst = i == np - 1 ? 0 : 1 + (i & 1);
+          use(pp);
+        }
+      }
+    }
+    p.in_bytes = p.out_bytes = ubytes;
   }
   w[0] = (w[0] + 255) / 256 * 256;
   w[1] = (w[1] + 255) / 256 * 256;
   p.work_bytes = w[0] + w[1];
   p.work0_bytes = w[0];
-  p.in_bytes = (size_t)p.fwd.front().in_elems * eltype_bytes(p.elt_in);
-  p.out_bytes = (size_t)p.fwd.back().out_elems * eltype_bytes(p.elt_out);
   return PFFT_OK;
 }
 

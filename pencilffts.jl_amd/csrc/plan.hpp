@@ -80,6 +80,7 @@ struct Plan {
   int32_t real_type = 1;     // PFFT_F32 / PFFT_F64
   bool permute = true;
   bool inplace = false;
+  int64_t pad_in0 = 0;     // in-place RFFT!: padded length 2 (n/2 + 1) of input memory axis 0
   bool adjoint_ok = true;  // no R2R stage: the adjoint directions are defined
   uint32_t flags = 0;
   double scale = 1.0;

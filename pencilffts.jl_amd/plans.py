@@ -54,10 +54,12 @@ class Pencil:
 class PencilArray:
     """A GPU array on a pencil.
 
-    `parent` is a contiguous torch tensor with the memory dims in reverse
-    (torch is row-major), so `parent.reshape(-1)` is the column-major memory
-    image of the Julia array.  `logical()` is a view indexed in logical order
-    (i1, i2, ..., e1, ...), like indexing a PencilArray.
+    `parent` is a torch tensor with the memory dims in reverse (torch is
+    row-major), so `parent.reshape(-1)` is the column-major memory image of
+    the Julia array.  It is contiguous, except for the real input view of an
+    in-place RFFT! plan, a slice of its padded parent.  `logical()` is a view
+    indexed in logical order (i1, i2, ..., e1, ...), like indexing a
+    PencilArray.
     """
 
     def __init__(self, pencil, eltype, parent):
@@ -102,6 +104,46 @@ class PencilArray:
 
     def __repr__(self):
         return "PencilArray{%s}(%s)" % (_NAMES[self.eltype], self.pencil)
+
+
+class ManyPencilArray:
+    """One GPU buffer viewed on the pencils of an in-place plan: PencilArrays'
+    ManyPencilArray, or ManyPencilArrayRFFT! for RFFT! (allocate.jl:57-83,
+    multiarrays_r2c.jl:26-79).
+
+    `first()` is the input view (the first pencil; for RFFT! real data whose
+    parent is padded to 2 (n/2 + 1) along memory axis 0, so it is a strided
+    view) and `last()` the output view (the last pencil); both start at the
+    buffer's first byte, and transforming in place moves the data from one to
+    the other."""
+
+    def __init__(self, plan, device="cuda"):
+        self.data = torch.empty(max(plan.in_bytes, 1), dtype=torch.uint8, device=device)
+        self.arrays = (self._view(plan.pencil_input, plan.eltype_input, plan._parent_dims(0)),
+                       self._view(plan.pencil_output, plan.eltype_output, plan._parent_dims(1)))
+
+    def _view(self, pencil, eltype, parent_dims):
+        dims = pencil.mem_dims()
+        t = self.data.view(_TORCH_DTYPES[eltype])[:int(np.prod(parent_dims))]
+        t = t.view(tuple(reversed(parent_dims)))
+        if tuple(parent_dims) != tuple(dims):
+            t = t[(Ellipsis, slice(0, dims[0]))]   # memory axis 0 is torch's last
+        return PencilArray(pencil, eltype, t)
+
+    def first(self):
+        return self.arrays[0]
+
+    def last(self):
+        return self.arrays[-1]
+
+    def data_ptr(self):
+        return self.data.data_ptr()
+
+    def nbytes(self):
+        return self.data.numel()
+
+    def __repr__(self):
+        return "ManyPencilArray(%s -> %s)" % (self.arrays[0], self.arrays[-1])
 
 
 def _mightalias(a, b):
@@ -191,6 +233,9 @@ class PencilFFTPlan:
         bi, bo, bw = C.c_size_t(), C.c_size_t(), C.c_size_t()
         L.check(L.lib.pfft_plan_nbytes(h, C.byref(bi), C.byref(bo), C.byref(bw)))
         self.in_bytes, self.out_bytes, self.work_bytes = bi.value, bo.value, bw.value
+        ip = C.c_int32()
+        L.check(L.lib.pfft_plan_is_inplace(h, C.byref(ip)))
+        self.inplace = bool(ip.value)
         self._work = None
 
     # ---- reference accessors (plans.jl:555-591) ----
@@ -199,7 +244,13 @@ class PencilFFTPlan:
         return self.size_global_in
 
     def is_inplace(self):
-        return False
+        return self.inplace
+
+    def _parent_dims(self, which):
+        nd = self.N + len(self.extra_dims)
+        d = (C.c_int64 * nd)()
+        L.check(L.lib.pfft_plan_parent_dims(self._h, which, d))
+        return tuple(d)
 
     def describe(self):
         buf = C.create_string_buffer(4096)
@@ -269,6 +320,8 @@ class PencilFFTPlan:
                                       "arrays must be contiguous GPU tensors")
 
     def _execute(self, direction, src, dst, stream=None):
+        if self.inplace:
+            return self._execute_inplace(direction, src, dst, stream)
         # mul! and the transpose of bmul! read the input pencil; bmul!, ldiv!
         # and the transpose of mul! read the output pencil
         if direction in (L.FORWARD, L.ADJOINT_BACKWARD):
@@ -284,19 +337,48 @@ class PencilFFTPlan:
                                    self.work_bytes, s.cuda_stream))
         return dst
 
+    def _execute_inplace(self, direction, src, dst, stream=None):
+        # operations.jl:100-116: the same ManyPencilArray in and out
+        if not isinstance(src, ManyPencilArray) or not isinstance(dst, ManyPencilArray):
+            raise L.ArgumentError(L.ERR_ARGUMENT, "array types (%s, %s) incompatible with "
+                                  "in-place plans" % (type(src).__name__, type(dst).__name__))
+        if src is not dst:
+            raise L.ArgumentError(L.ERR_ARGUMENT,
+                                  "input and output arrays for in-place plan must be the same")
+        if src.first().pencil != self.pencil_input or src.first().eltype != self.eltype_input:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of input data")
+        if src.last().pencil != self.pencil_output or src.last().eltype != self.eltype_output:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of output data")
+        if src.nbytes() < self.in_bytes or not src.data.is_cuda:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "buffer too small for the in-place plan")
+        w = self._workspace()
+        s = stream if stream is not None else torch.cuda.current_stream(src.data.device)
+        L.check(L.lib.pfft_execute(self._h, direction, src.data_ptr(), src.data_ptr(),
+                                   w.data_ptr() if w is not None else None,
+                                   self.work_bytes, s.cuda_stream))
+        return dst
+
+    def _maybe_allocate(self, allocator, src):
+        # operations.jl:73-85
+        if self.inplace and isinstance(src, ManyPencilArray):
+            return src
+        if not self.inplace and isinstance(src, PencilArray):
+            return allocator(self)
+        raise L.ArgumentError(L.ERR_ARGUMENT, "input array type %s incompatible with %s plans"
+                              % (type(src).__name__,
+                                 "in-place" if self.inplace else "out-of-place"))
+
     def __mul__(self, src):
-        """plan * u (operations.jl:54-57)."""
+        """plan * u (operations.jl:54-57); in place for in-place plans."""
         if isinstance(src, (tuple, list)):
             return type(src)(self * u for u in src)
-        dst = allocate_output(self)
-        return mul_(dst, self, src)
+        return mul_(self._maybe_allocate(allocate_output, src), self, src)
 
     def ldiv(self, src):
-        """plan \\ v (operations.jl:59-62)."""
+        """plan \\ v (operations.jl:59-62); in place for in-place plans."""
         if isinstance(src, (tuple, list)):
             return type(src)(self.ldiv(v) for v in src)
-        dst = allocate_input(self)
-        return ldiv_(dst, self, src)
+        return ldiv_(self._maybe_allocate(allocate_input, src), self, src)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -354,6 +436,8 @@ def allocate_input(p, *dims):
     count in a 1-tuple (allocate.jl:44-54,85-86)."""
     if dims:
         return _allocate_many(allocate_input, p, dims)
+    if p.inplace:
+        return ManyPencilArray(p, p.device)
     return PencilArray.empty(p.pencil_input, p.eltype_input, p.device)
 
 
@@ -361,6 +445,8 @@ def allocate_output(p, *dims):
     """allocate_output(p[, dims...]) (allocate.jl:100-110,115-122)."""
     if dims:
         return _allocate_many(allocate_output, p, dims)
+    if p.inplace:
+        return ManyPencilArray(p, p.device)
     return PencilArray.empty(p.pencil_output, p.eltype_output, p.device)
 
 

@@ -53,3 +53,42 @@ class Emulator:
             return out
         finally:
             self.lib.pfft_emu_plan_destroy(h)
+
+    def run_inplace(self, dims, kinds, real_type, directions, buf, permute=True, extra=()):
+        """In-place plan: runs each direction in turn on the uint8 buffer `buf`
+        (input and output views share it); returns the plan's buffer size."""
+        N = len(dims)
+        h = C.c_void_p()
+        ne = len(extra)
+        st = self.lib.pfft_emu_plan_create(
+            C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
+            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), 0)
+        if st:
+            raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+        try:
+            bi, bo = C.c_size_t(), C.c_size_t()
+            self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
+            assert bi.value == bo.value and buf.nbytes >= bi.value, (bi.value, bo.value)
+            for d in directions:
+                st = self.lib.pfft_emu_execute(h, d, buf.ctypes.data, buf.ctypes.data)
+                if st:
+                    raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+            return bi.value
+        finally:
+            self.lib.pfft_emu_plan_destroy(h)
+
+    def nbytes(self, dims, kinds, real_type, permute=True, extra=()):
+        N = len(dims)
+        h = C.c_void_p()
+        ne = len(extra)
+        st = self.lib.pfft_emu_plan_create(
+            C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
+            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), 0)
+        if st:
+            raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+        try:
+            bi, bo = C.c_size_t(), C.c_size_t()
+            self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
+            return bi.value, bo.value
+        finally:
+            self.lib.pfft_emu_plan_destroy(h)

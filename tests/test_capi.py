@@ -190,9 +190,12 @@ def test_error_contract(pf):
     # one GPU: proc_dims must be a single process
     with pytest.raises(pf.UnsupportedError, match="one GPU"):
         pf.PencilFFTPlan((16, 12, 6), T.FFT(), (2, 1), None, T.Float64)
-    # in-place plans are not provided (out of scope of this build)
-    with pytest.raises(pf.UnsupportedError, match="in-place"):
-        pf.PencilFFTPlan((16, 12, 6), T.FFT_(), (1,), None, T.Float64)
+    # in-place chains other than the reference's ManyPencilArray forms
+    # (allocate.jl:57-83): RFFT! must be followed by FFT!s, c2r is not in-place
+    with pytest.raises(pf.UnsupportedError, match="in-place plans take"):
+        pf.PencilFFTPlan((16, 12, 6), (T.RFFT_(), T.BFFT_(), T.FFT_()), (1,), None, T.Float64)
+    with pytest.raises(pf.UnsupportedError, match="in-place plans take"):
+        pf.PencilFFTPlan((16, 12, 6), T.BRFFT_(6), (1,), None, T.Float64)
     # REDFT00 needs length >= 2
     with pytest.raises(pf.ArgumentError, match="REDFT00"):
         pf.PencilFFTPlan((1, 4), T.R2R(T.REDFT00), (1,), None, T.Float64)
