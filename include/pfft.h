@@ -189,6 +189,47 @@ int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
                       const void* const* ins, void* const* outs, void* work,
                       size_t work_bytes, void* hip_stream);
 
+/* ---- spectral operators fused into the passes ------------------------------
+ * A diagonal multiplier alpha m(k) of the spectrum (the output of mul!), k the
+ * integer wavenumbers of each spectral element: rfftfreq along the r2c
+ * dimension, fftfreq along the c2c ones (test/include/Grids.jl:44-59 with a
+ * 2 pi box).  The forms are those of the reference's Fourier-space operations
+ * (test/include/FourierOperations.jl:18-86, docs/examples/gradient.jl:254-273,
+ * docs/examples/navier_stokes.jl:292-344):
+ *   PFFT_OP_DERIVATIVE     m = i k_dim                 (one gradient component)
+ *   PFFT_OP_LAPLACIAN      m = -|k|^2
+ *   PFFT_OP_INV_LAPLACIAN  m = -1/|k|^2, 0 at k = 0    (Poisson solve)
+ *   PFFT_OP_DEALIAS        m = 0 where any |k_d| > (2/3) max|k_d|, else 1
+ * pfft_execute_op fuses it into the pass that writes the spectrum
+ *   PFFT_FORWARD            out = alpha m F(in)
+ *   PFFT_ADJOINT_BACKWARD   out = alpha conj(m) B^T(in)
+ * or reads it
+ *   PFFT_BACKWARD[_SCALED]  out = alpha B(m in) [/ N]
+ *   PFFT_ADJOINT_FORWARD    out = alpha F^T(conj(m) in)
+ * so an operator costs no memory pass of its own.  On the writing side
+ * accumulate = 1 adds into `out` instead of overwriting it (sums over
+ * components, as in a divergence or a gradient's adjoint).  No counterpart in
+ * the reference's API: it applies these operators in separate Julia loops
+ * between transforms.  Plans need FFT/BFFT/RFFT dimensions, and the carrying
+ * pass must be a c2c one (two or more transformed dimensions), else
+ * PFFT_ERR_UNSUPPORTED.                                                      */
+enum {
+  PFFT_OP_NONE = 0,
+  PFFT_OP_DERIVATIVE = 1,
+  PFFT_OP_LAPLACIAN = 2,
+  PFFT_OP_INV_LAPLACIAN = 3,
+  PFFT_OP_DEALIAS = 4
+};
+typedef struct {
+  int32_t op;          /* PFFT_OP_* */
+  int32_t dim;         /* 0-based logical dimension of PFFT_OP_DERIVATIVE */
+  double alpha;        /* scalar factor */
+  int32_t accumulate;  /* writing side: out += result */
+} pfft_spectral_op;
+int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* op,
+                    const void* in, void* out, void* work, size_t work_bytes,
+                    void* hip_stream);
+
 /* Kernel timing of the last execute on a plan with profiling enabled: the
  * mirror of the reference's TimerOutputs sections "FFT" per stage
  * (operations.jl:250).  Enabled with pfft_plan_set_timing(plan, 1).        */

@@ -6,7 +6,7 @@ hand-written HIP kernels for gfx950 through the C ABI of include/pfft.h.
 """
 from . import _lib
 from . import transforms as Transforms
-from . import autodiff, batched
+from . import autodiff, batched, spectral
 from ._lib import ArgumentError, DeviceError, PfftError, UnsupportedError
 from .plans import (ManyPencilArray, Pencil, PencilArray, PencilFFTPlan, Val, allocate_input,
                     allocate_output, bmul_, execute_many, ldiv_, mul_, scale_factor)
@@ -15,6 +15,6 @@ __all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "ManyPencilArray", "Pen
            "allocate_input",
            "allocate_output", "mul_", "ldiv_", "bmul_", "scale_factor", "Val",
            "execute_many", "ArgumentError", "UnsupportedError", "DeviceError", "PfftError",
-           "autodiff", "batched"]
+           "autodiff", "batched", "spectral"]
 
 __version__ = _lib.lib.pfft_version().decode()

@@ -20,6 +20,15 @@ def FLAG_CANDIDATE(k):
     """PFFT_FLAG_CANDIDATE(k): run candidate k of every specialised pass."""
     return ((int(k) + 1) & 0xff) << 16
 
+OP_NONE, OP_DERIVATIVE, OP_LAPLACIAN, OP_INV_LAPLACIAN, OP_DEALIAS = 0, 1, 2, 3, 4
+
+
+class SpectralOp(C.Structure):
+    """pfft_spectral_op: a multiplier alpha m(k) fused into pfft_execute_op."""
+    _fields_ = [("op", C.c_int32), ("dim", C.c_int32), ("alpha", C.c_double),
+                ("accumulate", C.c_int32)]
+
+
 # every symbol include/pfft.h declares, with (restype, argtypes)
 i32, i64, u32, sz, dbl, vp, cp = (C.c_int32, C.c_int64, C.c_uint32, C.c_size_t, C.c_double,
                                   C.c_void_p, C.c_char_p)
@@ -52,6 +61,7 @@ SIGNATURES = {
     "pfft_plan_stage": (C.c_int, [vp, i32, P(i32), P(i64), P(i32), P(i64), P(i32)]),
     "pfft_plan_describe": (C.c_int, [vp, C.c_char_p, sz]),
     "pfft_execute": (C.c_int, [vp, i32, vp, vp, vp, sz, vp]),
+    "pfft_execute_op": (C.c_int, [vp, i32, C.POINTER(SpectralOp), vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
     "pfft_plan_set_timing": (C.c_int, [vp, i32]),
     "pfft_plan_kernels": (C.c_int, [vp, i32, C.c_char_p, sz]),

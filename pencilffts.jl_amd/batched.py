@@ -18,15 +18,14 @@ with wavenumbers laid out as in test/include/Grids.jl:44-59 (rfftfreq along
 the r2c dimension, fftfreq elsewhere), and its gradient by the hand-written
 adjoint chain:
 
-    g_bar   = 2 g
-    gh_bar  = (ldiv!)^T g_bar            PFFT_ADJOINT_BACKWARD, / N
-    uh_bar  = sum_j conj(i k_j) gh_bar_j
-    u_bar   = (mul!)^T uh_bar            PFFT_ADJOINT_FORWARD
+    g_j     = ldiv!(i k_j u^)                 ldiv_ with the derivative operator
+    u^_bar  = sum_j (2/N) conj(i k_j) B^T g_j  transpose of bmul! with the operator,
+                                              accumulated over j
+    u_bar   = (mul!)^T u^_bar                 PFFT_ADJOINT_FORWARD
 
-The constant factors of the chain (i, 2, 1/N, -i) are folded into two sets
-of complex multipliers, c_j = i k_j for the derivative and
-cb_j = -2 i k_j / N for the gradient, so each elementwise step is one
-torch kernel.  No reduction runs inside the captured graph: the loss sum is
+The multipliers i k_j and their conjugates ride on the passes that read or
+write the spectrum (pfft_execute_op), so no elementwise kernel runs between
+transforms.  No reduction runs inside the captured graph: the loss sum is
 taken after a replay.
 """
 import math
@@ -34,11 +33,9 @@ import math
 import torch
 
 from . import _lib as L
+from . import spectral as S
 from . import transforms as Tr
 from .plans import PencilArray, PencilFFTPlan
-
-_REAL = {L.F32: torch.float32, L.F64: torch.float64}
-_CPLX = {L.F32: torch.complex64, L.F64: torch.complex128}
 
 
 class _Group:
@@ -46,62 +43,42 @@ class _Group:
         self.edge = edge
         self.n = nfields
         dims = (edge, edge, edge)
-        # input: u[field, component, z, y, x] (memory order x, y, z, c, f)
-        self.plan_u = PencilFFTPlan(dims, Tr.RFFT(), (1,), None, real_type,
-                                    extra_dims=(3, nfields), device=device)
-        # derivatives: g[field, component, direction, ...] (memory x, y, z, j, c, f)
-        self.plan_g = PencilFFTPlan(dims, Tr.RFFT(), (1,), None, real_type,
-                                    extra_dims=(3, 3, nfields), device=device)
-        for p in (self.plan_u, self.plan_g):
-            L.check(L.lib.pfft_plan_prepare(p._h, None))
-            p._workspace()
-        # wavenumbers on the output layout: parent dims (f, [c,] [j,] x', y, z)
-        X = edge // 2 + 1
-        kx = torch.arange(X, dtype=_REAL[real_type], device=device)
-        kf = torch.fft.fftfreq(edge, 1.0 / edge, dtype=_REAL[real_type], device=device)
-        self.k = [kx.view(X, 1, 1), kf.view(1, edge, 1), kf.view(1, 1, edge)]
-        ct = _CPLX[real_type]
-        N = self.plan_g.scale_factor
-        self.c = [(1j * k).to(ct) for k in self.k]                   # d/dx_j
-        self.cb = [(-2j / N * k).to(ct) for k in self.k]            # gradient chain
+        # u[field, component, z, y, x] (memory order x, y, z, c, f)
+        self.plan = PencilFFTPlan(dims, Tr.RFFT(), (1,), None, real_type,
+                                  extra_dims=(3, nfields), device=device)
+        p = self.plan
+        L.check(L.lib.pfft_plan_prepare(p._h, None))
+        p._workspace()
+        new_in = lambda: PencilArray.empty(p.pencil_input, p.eltype_input, device)
+        new_out = lambda: PencilArray.empty(p.pencil_output, p.eltype_output, device)
         # static buffers (the graph's inputs and outputs)
-        self.u = PencilArray.empty(self.plan_u.pencil_input, self.plan_u.eltype_input, device)
-        self.uh = PencilArray.empty(self.plan_u.pencil_output, self.plan_u.eltype_output, device)
-        self.gh = PencilArray.empty(self.plan_g.pencil_output, self.plan_g.eltype_output, device)
-        self.g = PencilArray.empty(self.plan_g.pencil_input, self.plan_g.eltype_input, device)
-        self.ghb = PencilArray.empty(self.plan_g.pencil_output, self.plan_g.eltype_output,
-                                     device)
-        self.uhb = PencilArray.empty(self.plan_u.pencil_output, self.plan_u.eltype_output,
-                                     device)
-        self.ub = PencilArray.empty(self.plan_u.pencil_input, self.plan_u.eltype_input, device)
+        self.u, self.uh, self.uhb, self.ub = new_in(), new_out(), new_out(), new_in()
+        self.g = [new_in() for _ in range(3)]     # d_j u, j = x, y, z
+        self.alpha = 2.0 / p.scale_factor
 
     def run(self):
         """Enqueue loss terms and gradient on the current stream."""
-        pu, pg = self.plan_u, self.plan_g
-        pu._execute(L.FORWARD, self.u, self.uh)                         # u^ = RFFT(u)
-        uh = self.uh.parent                                             # (f, c, x', y, z)
-        gh = self.gh.parent                                             # (f, c, j, x', y, z)
+        p = self.plan
+        p._execute(L.FORWARD, self.u, self.uh)                          # u^ = RFFT(u)
         for j in range(3):
-            torch.mul(uh, self.c[j], out=gh[:, :, j])                   # i k_j u^
-        pg._execute(L.BACKWARD_SCALED, self.gh, self.g)                 # g = ldiv(i k u^)
-        # dL/dg = 2 g; (ldiv!)^T = (bmul!)^T / N; the 2 / N ride in cb_j
-        pg._execute(L.ADJOINT_BACKWARD, self.g, self.ghb)
-        ghb = self.ghb.parent
-        uhb = self.uhb.parent
-        torch.mul(ghb[:, :, 0], self.cb[0], out=uhb)                    # sum_j conj(i k_j) .
-        uhb.addcmul_(ghb[:, :, 1], self.cb[1]).addcmul_(ghb[:, :, 2], self.cb[2])
-        pu._execute(L.ADJOINT_FORWARD, self.uhb, self.ub)               # (mul!)^T
+            S.ldiv_(self.g[j], p, self.uh, S.DERIVATIVE, dim=j)         # g_j = ldiv(i k_j u^)
+        for j in range(3):                                              # dL/du^
+            S.bmul_adjoint_(self.uhb, p, self.g[j], S.DERIVATIVE, dim=j, alpha=self.alpha,
+                            accumulate=j > 0)
+        p._execute(L.ADJOINT_FORWARD, self.uhb, self.ub)                # (mul!)^T
 
     def loss(self):
-        return torch.sum(self.g.parent.double() ** 2)
+        return sum(torch.sum(g.parent.double() ** 2) for g in self.g)
 
     def grad_bytes(self):
-        """Algorithmic HBM bytes of the six plan executions of one run()."""
+        """Algorithmic HBM bytes of the plan executions of one run(): eight
+        three-pass executions over 3n components (R + 5C each) and the reads
+        of the two accumulating stores."""
         e = self.edge
-        R = e ** 3 * self.g.parent.element_size()
-        C = (e // 2 + 1) * e * e * 2 * self.g.parent.element_size()
-        per_dir = R + 5 * C           # r2c/c2r pass + two c2c passes
-        return per_dir * (3 * self.n) * 2 + per_dir * (9 * self.n) * 2
+        rb = self.u.parent.element_size()
+        R = e ** 3 * rb
+        C = (e // 2 + 1) * e * e * 2 * rb
+        return (3 * self.n) * (8 * (R + 5 * C) + 2 * C)
 
 
 class GradientEnergy:
@@ -125,6 +102,11 @@ class GradientEnergy:
         """Logical view u[x, y, z, c] of field i (a view into the group buffer)."""
         g, j = self.slot[i]
         return g.u.parent[j].permute(3, 2, 1, 0)
+
+    def derivative(self, i, j):
+        """Logical view d_j u[x, y, z, c] of field i after run()."""
+        g, k = self.slot[i]
+        return g.g[j].parent[k].permute(3, 2, 1, 0)
 
     def grad(self, i):
         g, j = self.slot[i]

@@ -261,6 +261,14 @@ int pfft_execute(pfft_plan* plan, int32_t direction, const void* in, void* out, 
                                    (hipStream_t)hip_stream);)
 }
 
+int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* op,
+                    const void* in, void* out, void* work, size_t work_bytes, void* hip_stream) {
+  NEED(plan);
+  NEED(op);
+  PFFT_GUARD(return device_execute(plan->p, direction, in, out, work, work_bytes,
+                                   (hipStream_t)hip_stream, op->op ? op : nullptr);)
+}
+
 int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
                       const void* const* ins, void* const* outs, void* work, size_t work_bytes,
                       void* hip_stream) {

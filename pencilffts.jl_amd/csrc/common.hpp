@@ -73,6 +73,22 @@ struct PassArgs {
   double scale;        // output multiplier (1 unless ldiv! is fused)
   int32_t do_scale;
   int32_t adjoint;     // fuse the Hermitian weights of the r2c/c2r adjoints
+  // Spectral operator fused into this pass (pfft_execute_op): the elements it
+  // reads (op_side 1) or writes (op_side 2) are multiplied by alpha m(k), k
+  // the integer wavenumbers of the spectrum element; op_acc adds the written
+  // value into the output.
+  int32_t op;          // PFFT_OP_* (0: none)
+  int32_t op_side;
+  int32_t op_dim;      // logical dim of PFFT_OP_DERIVATIVE
+  int32_t op_conj;     // conj(m): the transposes
+  int32_t op_acc;
+  int32_t pdim;        // logical dim transformed by this pass
+  int32_t kN;          // logical dims of the spectrum
+  int32_t llog[kMaxDims];   // logical dim of each line dim (-1: extra dims)
+  int32_t kmode[kMaxDims];  // per logical dim: 0 fftfreq, 1 rfftfreq
+  int64_t kn[kMaxDims];     // per logical dim: spectrum length
+  double kmax[kMaxDims];    // per logical dim: largest stored |k|
+  double op_alpha;
 };
 
 template <typename R>

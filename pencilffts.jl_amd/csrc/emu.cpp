@@ -68,10 +68,17 @@ __attribute__((visibility("default"))) int pfft_emu_nbytes(const pfft_emu_plan* 
   return PFFT_OK;
 }
 
-// Host buffers in the same layouts as pfft_execute's device buffers.
-__attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, int32_t dir,
-                                                           const void* in, void* out) {
+// Host buffers in the same layouts as pfft_execute's device buffers; op as
+// in pfft_execute_op (null: none).
+__attribute__((visibility("default"))) int pfft_emu_execute_op(pfft_emu_plan* h, int32_t dir,
+                                                              const pfft_spectral_op* op,
+                                                              const void* in, void* out) {
   Plan& p = h->p;
+  int op_pass = -1, op_side = 0;
+  if (op && op->op) {
+    const int r0 = op_setup(p, dir, *op, &op_pass, &op_side);
+    if (r0) return r0;
+  }
   if (dir < PFFT_FORWARD || dir > PFFT_ADJOINT_BACKWARD)
     return fail(PFFT_ERR_ARGUMENT, "unknown direction");
   if ((dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) && !p.adjoint_ok)
@@ -96,10 +103,16 @@ __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, in
       a.scale = 1.0 / p.scale;
     }
     a.adjoint = (dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) ? 1 : 0;
+    if (i == op_pass) op_fill(p, dir, *op, op_side, a);
     if (p.real_type == PFFT_F32) emu_pass<float>(pp, a);
     else emu_pass<double>(pp, a);
   }
   return PFFT_OK;
+}
+
+__attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, int32_t dir,
+                                                           const void* in, void* out) {
+  return pfft_emu_execute_op(h, dir, nullptr, in, out);
 }
 
 }  // extern "C"

@@ -483,10 +483,25 @@ __device__ __forceinline__ int tile_idx(int k, int t) {
 // with compile-time trip counts, so a thread issues all its global loads
 // before the first LDS write; the partial last round is a separate guarded
 // step that keeps nothing live across its branch.
-template <typename R, int T, int ROWS, int NTHR>
+template <typename R, int T, int ROWS, int NTHR, bool ACC = false>
 __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, int64_t base,
                                            int tid) {
   C<R>* out = (C<R>*)a.out;
+  // ACC: add into the output (spectral operators with accumulate)
+  auto put = [&](float4* dst, float4 v) {
+    if constexpr (ACC) {
+      const float4 o = *dst;
+      if constexpr (sizeof(R) == 4) {
+        v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+      } else {
+        double2 vd = *(double2*)&v;
+        const double2 od = *(const double2*)&o;
+        vd.x += od.x; vd.y += od.y;
+        v = *(float4*)&vd;
+      }
+    }
+    *dst = v;
+  };
   constexpr int EPV = sizeof(R) == 4 ? 2 : 1;      // complex per 16-byte vector
   constexpr int VPR = T / EPV;                     // vectors per row
   constexpr int TOTAL = ROWS * VPR;
@@ -496,16 +511,16 @@ __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, 
     const int e = tid + i * NTHR;
     const int k = e / VPR;
     const int t = EPV * (e - k * VPR);
-    *(float4*)&out[base + (int64_t)k * a.sk_out + t] =
-        *(const float4*)&tile[tile_idx<R, T>(k, t)];
+    put((float4*)&out[base + (int64_t)k * a.sk_out + t],
+        *(const float4*)&tile[tile_idx<R, T>(k, t)]);
   }
   if constexpr (TOTAL % NTHR != 0) {
     const int e = tid + FULL * NTHR;
     if (e < TOTAL) {
       const int k = e / VPR;
       const int t = EPV * (e - k * VPR);
-      *(float4*)&out[base + (int64_t)k * a.sk_out + t] =
-          *(const float4*)&tile[tile_idx<R, T>(k, t)];
+      put((float4*)&out[base + (int64_t)k * a.sk_out + t],
+          *(const float4*)&tile[tile_idx<R, T>(k, t)]);
     }
   }
 }
@@ -555,8 +570,11 @@ constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per li
 // OCC > 0 asks the compiler for at least OCC waves per SIMD (a register
 // budget of 512 / OCC): more waves in flight at the price of possible spills;
 // the autotuner keeps such a variant only where it is faster.
+// OPK = 1 compiles in the spectral operator of pfft_execute_op (c2c only):
+// the elements read (op_side 1) or written (op_side 2) are multiplied by
+// alpha m(k); those variants run only the passes that carry an operator.
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
-          int OCC>
+          int OCC, int OPK>
 __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L))
 __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
@@ -564,6 +582,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   constexpr int NTHR = TQ * L;
   static_assert(TQ * LPW == T, "LPW must divide T");
   static_assert(P % NSP == 0 && (NSP == 1 || ST == IO_TILE), "store chunks split P");
+  static_assert(OPK == 0 || KIND == K_C2C, "spectral operators ride on c2c passes");
   // inputs / outputs per line (the r2c output and c2r input carry M + 1)
   constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
   constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
@@ -670,9 +689,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           v[q][p] = conj_in ? conj(z) : z;
         }
       } else if constexpr (LD == IO_TILE) {
+        OpLine ol;
+        if constexpr (OPK) {
+          int64_t m[kMaxDims];
+          line_index(a, tl, t < c.Tact ? t : 0, m);
+          ol = op_line(a, m);
+        }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          const C<R> x = TK(p, t, 0);
+          C<R> x = TK(p, t, 0);
+          if constexpr (OPK)
+            if (a.op_side == 1) x = x * op_factor<R>(a, ol, l + L * p);
           v[q][p] = conj_in ? conj(x) : x;
         }
       } else {
@@ -681,9 +708,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
                              ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
                              : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
         if (active) {
+          OpLine ol;
+          if constexpr (OPK) {
+            int64_t m[kMaxDims];
+            line_index(a, tl, t, m);
+            ol = op_line(a, m);
+          }
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const C<R> x = in[l + L * p];
+            C<R> x = in[l + L * p];
+            if constexpr (OPK)
+              if (a.op_side == 1) x = x * op_factor<R>(a, ol, l + L * p);
             v[q][p] = conj_in ? conj(x) : x;
           }
         }
@@ -766,11 +801,25 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       } else {
 #pragma unroll
         for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], scq);
+        if constexpr (OPK) {
+          if (a.op_side == 2) {
+            int64_t m[kMaxDims];
+            line_index(a, tl, active ? t : 0, m);
+            const OpLine ol = op_line(a, m);
+#pragma unroll
+            for (int p = 0; p < P; ++p) v[q][p] = v[q][p] * op_factor<R>(a, ol, l + L * p);
+          }
+        }
         if constexpr (ST == IO_LINE) {
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
+            if (OPK && a.op_acc) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p];
+              for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p] + out[l + L * p];
+            } else {
+#pragma unroll
+              for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p];
+            }
           }
         }
       }
@@ -794,8 +843,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         }
         __syncthreads();
         const int64_t ob = c.out_base + (int64_t)(h * RCH) * a.sk_out;
-        if (h == NSP - 1) tile_store<R, T, ROWS_OUT - (NSP - 1) * RCH, NTHR>(tile, a, ob, tid);
-        else tile_store<R, T, RCH, NTHR>(tile, a, ob, tid);
+        constexpr int LAST_ROWS = ROWS_OUT - (NSP - 1) * RCH;
+        if (OPK && a.op_acc) {
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, true>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR, true>(tile, a, ob, tid);
+        } else {
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR>(tile, a, ob, tid);
+        }
       }
     }
     __syncthreads();
@@ -804,20 +859,20 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w, occ;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w, occ, opk;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
-          int OCC>
+          int OCC, int OPK>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>), g, b, lds, s, a);
 }
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1,
-          int W = 1, int OCC = 0>
+          int W = 1, int OCC = 0, int OPK = 0>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -831,10 +886,11 @@ static void add(std::vector<Entry>& v) {
   e.nsp = NSP;
   e.w = W;
   e.occ = OCC;
+  e.opk = OPK;
   e.threads = T / LPW * Cfg<M, W>::L;
   e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC>;
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>;
   v.push_back(e);
 }
 
@@ -854,6 +910,17 @@ static void add_mfma(std::vector<Entry>& v) {
   add<float, 1024, 16, 4, K_C2C, IO_LINE, IO_TILE, MF, 2, 2>(v);
   add<float, 1024, 8, 2, K_C2C, IO_LINE, IO_TILE, MF, 2, 2>(v);
   add<float, 1024, 8, 2, K_C2C, IO_TILE, IO_LINE, MF, 1, 2>(v);
+}
+
+// c2c passes that carry a spectral operator (pfft_execute_op): one tile shape
+// per load/store form
+template <typename R, int M>
+static void add_op(std::vector<Entry>& v) {
+  constexpr int TL = 256 / Cfg<M>::L;
+  constexpr int TT = sizeof(R) == 4 && M < 1024 ? 16 : 8;
+  add<R, M, TL, 1, K_C2C, IO_LINE, IO_LINE, 0, 1, 1, 0, 1>(v);
+  add<R, M, TT, 1, K_C2C, IO_LINE, IO_TILE, 0, 1, 1, 0, 1>(v);
+  add<R, M, TT, 1, K_C2C, IO_TILE, IO_LINE, 0, 1, 1, 0, 1>(v);
 }
 
 template <typename R, int M>
@@ -963,6 +1030,19 @@ static const std::vector<Entry>& registry() {
     add_size<double, 128>(r);
     add_size<double, 256>(r);
     add_size<double, 512>(r);
+    add_op<float, 64>(r);
+    add_op<float, 96>(r);
+    add_op<float, 128>(r);
+    add_op<float, 192>(r);
+    add_op<float, 256>(r);
+    add_op<float, 320>(r);
+    add_op<float, 384>(r);
+    add_op<float, 512>(r);
+    add_op<float, 1024>(r);
+    add_op<double, 64>(r);
+    add_op<double, 128>(r);
+    add_op<double, 256>(r);
+    add_op<double, 512>(r);
     return r;
   }();
   return v;
@@ -979,7 +1059,7 @@ static bool all_even(const int64_t* s, int from, int D) {
 }
 
 // Specialised kernels that can run a pass (first = default choice).
-std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp) {
+std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
   std::vector<int> out;
   if (p.flags & PFFT_FLAG_GENERIC_ONLY) return out;
   const PassArgs& a = pp.args;
@@ -1014,6 +1094,7 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp) {
     const Entry& x = reg[i];
     if (x.real_bytes != rb || x.M != M || x.kind != kind || x.ld != ld || x.st != st) continue;
     if (x.mf != want_mf) continue;
+    if (x.opk != (op ? 1 : 0)) continue;
     if (x.lds > 160 * 1024) continue;
     if (ld == IO_TILE || st == IO_TILE) {
       // transposed tiles: full tiles, 16-byte aligned runs
@@ -1038,6 +1119,7 @@ void fast_apply(PassPlan& pp, int variant) {
         std::swap(a.ldim[0], a.ldim[d]);
         std::swap(a.lsin[0], a.lsin[d]);
         std::swap(a.lsout[0], a.lsout[d]);
+        std::swap(a.llog[0], a.llog[d]);
         break;
       }
     }
@@ -1085,6 +1167,7 @@ const char* fast_name(int variant) {
       std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
       (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
+      (e.opk ? ",op" : "") +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

@@ -6,7 +6,8 @@
 namespace pfft {
 
 // Specialised kernels able to run a pass, as variant ids (first = default).
-std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp);
+// op: variants that carry a spectral operator (pfft_execute_op)
+std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op = false);
 // Switches a pass to a specialised variant (tile shape, grid, LDS).
 void fast_apply(PassPlan& pp, int variant);
 int fast_prepare(PassPlan& pp);

@@ -20,9 +20,77 @@ namespace pfft {
 struct TileCtx {
   int64_t in_base;   // element offset of the tile's first line (input)
   int64_t out_base;  // element offset of the tile's first line (output)
+  int64_t tile;      // tile index
   int32_t Tact;      // lines present in this tile
   int32_t Tw;        // work lines (two per line for complex R2R data)
 };
+
+// Line multi-index (over the line dims) of line t of a tile.
+PFFT_HD void line_index(const PassArgs& a, int64_t tile, int t, int64_t* m) {
+  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
+  m[0] = (tile % nb0) * a.T + t;
+  int64_t rest = tile / nb0;
+  for (int d = 1; d < a.D; ++d) {
+    m[d] = rest % a.ldim[d];
+    rest /= a.ldim[d];
+  }
+}
+
+// Integer wavenumber of index c along logical dim d of the spectrum:
+// rfftfreq along the r2c dim, fftfreq elsewhere (test/include/Grids.jl:44-59
+// with a 2 pi box).
+PFFT_HD double wavenumber(const PassArgs& a, int d, int64_t c) {
+  const int64_t n = a.kn[d];
+  return a.kmode[d] == 1 ? (double)c : (double)(c < (n + 1) / 2 ? c : c - n);
+}
+
+// Per-line part of a spectral operator: the wavenumbers of every logical dim
+// but the pass's own, folded into (A, B): m(k) = f(A, B, k_pass).
+struct OpLine {
+  double a, b;
+};
+
+PFFT_HD OpLine op_line(const PassArgs& a, const int64_t* m) {
+  double k[kMaxDims];
+  for (int d = 0; d < a.kN; ++d) k[d] = 0.0;
+  for (int d = 0; d < a.D; ++d)
+    if (a.llog[d] >= 0) k[a.llog[d]] = wavenumber(a, a.llog[d], m[d]);
+  OpLine o = {0.0, 1.0};
+  switch (a.op) {
+    case 1:   // i k_dim: the dim's wavenumber when it is not the pass's
+      o.a = a.op_dim == a.pdim ? 0.0 : k[a.op_dim];
+      break;
+    case 2:
+    case 3:   // |k|^2 over the other dims
+      for (int d = 0; d < a.kN; ++d)
+        if (d != a.pdim) o.a += k[d] * k[d];
+      break;
+    case 4:   // 2/3 rule over the other dims (navier_stokes.jl:331-344)
+      for (int d = 0; d < a.kN; ++d)
+        if (d != a.pdim && fabs(k[d]) > (2.0 / 3.0) * a.kmax[d]) o.b = 0.0;
+      break;
+    default:
+      break;
+  }
+  return o;
+}
+
+// alpha m(k) (conjugated for the transposes) at position j along the pass dim.
+template <typename R>
+PFFT_HD cpx<R> op_factor(const PassArgs& a, const OpLine& o, int64_t j) {
+  const double kp = wavenumber(a, a.pdim, j);
+  double re = 0.0, im = 0.0;
+  switch (a.op) {
+    case 1: im = a.op_dim == a.pdim ? kp : o.a; break;
+    case 2: re = -(o.a + kp * kp); break;
+    case 3: { const double s = o.a + kp * kp; re = s > 0.0 ? -1.0 / s : 0.0; break; }
+    case 4: re = (o.b != 0.0 && fabs(kp) <= (2.0 / 3.0) * a.kmax[a.pdim]) ? 1.0 : 0.0; break;
+    default: re = 1.0; break;
+  }
+  re *= a.op_alpha;
+  im *= a.op_alpha;
+  return mk(R(re), R(a.op_conj ? -im : im));
+}
 
 PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
   const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
@@ -40,6 +108,7 @@ PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
   c.out_base = ob;
   const int64_t left = a.ldim[0] - b0 * a.T;
   c.Tact = (int32_t)(left < a.T ? left : a.T);
+  c.tile = tile;
   c.Tw = a.r2r_cplx ? 2 * c.Tact : c.Tact;
 }
 
@@ -96,6 +165,11 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
     switch (a.kind) {
       case L_C2C:
         v = raw_cplx(a, raw, w, j);
+        if (a.op && a.op_side == 1) {
+          int64_t m[kMaxDims];
+          line_index(a, c.tile, (int)w, m);
+          v = v * op_factor<R>(a, op_line(a, m), j);
+        }
         break;
       case L_R2C_HALF:
         v = mk(raw_real(a, raw, w, 2 * j), raw_real(a, raw, w, 2 * j + 1));
@@ -386,6 +460,12 @@ PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       else if (a.r2r_cplx) y = mk(post_real(a, W, 2 * t, k), post_real(a, W, 2 * t + 1, k));
       else y = post_cplx(a, W, t, k);
       if (a.do_scale) y = scl(y, sc);
+      if (a.op && a.op_side == 2) {
+        int64_t m[kMaxDims];
+        line_index(a, c.tile, (int)t, m);
+        y = y * op_factor<R>(a, op_line(a, m), k);
+        if (a.op_acc) y = y + ((const cpx<R>*)a.out)[g];
+      }
       ((cpx<R>*)a.out)[g] = y;
     } else {
       R y = a.kind == L_COPY ? raw[t * a.raw_pitch + k] : post_real(a, W, t, k);

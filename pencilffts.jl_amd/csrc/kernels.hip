@@ -291,7 +291,7 @@ static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
 }
 
 int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
-                   hipStream_t stream) {
+                   hipStream_t stream, const pfft_spectral_op* op) {
   if (dir < PFFT_FORWARD || dir > PFFT_ADJOINT_BACKWARD)
     return fail(PFFT_ERR_ARGUMENT, "unknown direction " + std::to_string(dir));
   if (!in || !out) return fail(PFFT_ERR_ARGUMENT, "null array pointer");
@@ -309,6 +309,11 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
       return fail(PFFT_ERR_ARGUMENT, "input and output arrays for in-place plan must be the same");
   } else if (overlaps(in, nin, out, nout)) {
     return fail(PFFT_ERR_ARGUMENT, "out-of-place plan applied to aliased data");
+  }
+  int op_pass = -1, op_side = 0;
+  if (op) {
+    const int r0 = op_setup(p, dir, *op, &op_pass, &op_side);
+    if (r0) return r0;
   }
   int r = device_prepare(p);
   if (r) return r;
@@ -339,8 +344,26 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
   const int np = (int)passes.size();
   if (d->timing) HIP_TRY(hipEventRecord(d->ev[0], stream));
   for (int i = 0; i < np; ++i) {
-    PassArgs a = passes[i].args;
-    const PassPlan& pp = passes[i];
+    PassPlan opp;   // the operator-carrying form of this pass
+    if (i == op_pass) {
+      opp = passes[i];
+      opp.args = opp.gen_args;
+      opp.threads = opp.gen_threads;
+      opp.lds_bytes = opp.gen_lds;
+      opp.grid = opp.gen_grid;
+      opp.variant = 0;
+      if (passes[i].variant != 0) {
+        const std::vector<int> c = fast_candidates(p, opp, true);
+        if (!c.empty()) {
+          fast_apply(opp, c[0]);
+          r = fast_prepare(opp);
+          if (r) return r;
+        }
+      }
+      op_fill(p, dir, *op, op_side, opp.args);
+    }
+    const PassPlan& pp = i == op_pass ? opp : passes[i];
+    PassArgs a = pp.args;
     a.in = pp.src == 0 ? in : bufs[pp.src];
     a.out = pp.dst == 0 ? out : bufs[pp.dst];
     a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;

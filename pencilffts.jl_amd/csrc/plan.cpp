@@ -451,7 +451,9 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
     a.ldim[i] = keep[i].len;
     a.lsin[i] = keep[i].si;
     a.lsout[i] = keep[i].so;
+    a.llog[i] = (keep[i].logical >= 0 && keep[i].logical < N) ? keep[i].logical : -1;
   }
+  a.pdim = dim;
 
   // tile size and LDS footprint of the generic pass
   const int64_t rs = p.real_type == PFFT_F32 ? 4 : 8;
@@ -488,7 +490,56 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
   pp.grid = std::min<int64_t>(nt, 1 << 20);
   a.scale = 1.0;
   a.do_scale = 0;
+  pp.gen_args = a;
+  pp.gen_threads = pp.threads;
+  pp.gen_lds = pp.lds_bytes;
+  pp.gen_grid = pp.grid;
   return PFFT_OK;
+}
+
+// ---- spectral operators (pfft_execute_op) ------------------------------------
+int op_setup(const Plan& p, int dir, const pfft_spectral_op& op, int* pass, int* side) {
+  if (op.op < PFFT_OP_DERIVATIVE || op.op > PFFT_OP_DEALIAS)
+    return fail(PFFT_ERR_ARGUMENT, "unknown spectral operator " + std::to_string(op.op));
+  if (op.op == PFFT_OP_DERIVATIVE && (op.dim < 0 || op.dim >= p.N))
+    return fail(PFFT_ERR_ARGUMENT, "derivative dimension out of range");
+  for (int d = 0; d < p.N; ++d) {
+    const int32_t b = kind_base(p.kinds[d]);
+    if (b != PFFT_FFT && b != PFFT_BFFT && b != PFFT_RFFT)
+      return fail(PFFT_ERR_UNSUPPORTED,
+                  "spectral operators need wavenumbers: FFT/BFFT/RFFT dimensions only");
+  }
+  // mul! and the transpose of bmul! write the spectrum in the last forward
+  // pass; bmul!/ldiv! and the transpose of mul! read it in the first
+  // backward pass
+  const bool writes = dir == PFFT_FORWARD || dir == PFFT_ADJOINT_BACKWARD;
+  const std::vector<PassPlan>& list = writes ? p.fwd : p.bwd;
+  *pass = writes ? (int)list.size() - 1 : 0;
+  *side = writes ? 2 : 1;
+  if (op.accumulate && !writes)
+    return fail(PFFT_ERR_ARGUMENT,
+                "accumulate applies where the spectrum is written (forward, adjoint of bmul!)");
+  if (list[*pass].args.kind != L_C2C)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "the pass carrying a spectral operator must be a c2c one (two or more "
+                "transformed dimensions)");
+  return PFFT_OK;
+}
+
+void op_fill(const Plan& p, int dir, const pfft_spectral_op& op, int side, PassArgs& a) {
+  a.op = op.op;
+  a.op_side = side;
+  a.op_dim = op.dim;
+  a.op_conj = (dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) ? 1 : 0;
+  a.op_acc = op.accumulate ? 1 : 0;
+  a.op_alpha = op.alpha;
+  a.kN = p.N;
+  for (int d = 0; d < p.N; ++d) {
+    const bool r2c = kind_base(p.kinds[d]) == PFFT_RFFT;
+    a.kmode[d] = r2c ? 1 : 0;
+    a.kn[d] = p.size_out[d];
+    a.kmax[d] = r2c ? (double)(p.size_out[d] - 1) : (double)(p.size_out[d] / 2);
+  }
 }
 
 // ---- plan construction -----------------------------------------------------

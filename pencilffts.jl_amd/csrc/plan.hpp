@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/pfft.h"
 #include "common.hpp"
 
 namespace pfft {
@@ -48,6 +49,11 @@ struct PassPlan {
   int64_t grid;
   int32_t src;              // 0 = user input, 1 = work0, 2 = work1
   int32_t dst;              // 0 = user output, 1 = work0, 2 = work1
+  // the generic form of the pass as built (before a specialised variant was
+  // applied): the starting point of the operator-carrying variants
+  PassArgs gen_args;
+  int32_t gen_threads;
+  int64_t gen_lds, gen_grid;
 };
 
 struct Table {
@@ -95,13 +101,20 @@ struct Plan {
 int plan_build(Plan& p, int32_t ndims, const int64_t* dims, const int32_t* kinds,
                int32_t real_type, int32_t permute, int32_t n_extra,
                const int64_t* extra, uint32_t flags);
+
+// Spectral operator of pfft_execute_op: checks it against the plan and the
+// direction, and names the pass that carries it (index in that direction's
+// pass list) and its side (1: on the elements read, 2: on those written).
+int op_setup(const Plan& p, int dir, const pfft_spectral_op& op, int* pass, int* side);
+// Writes the operator into the arguments of the carrying pass.
+void op_fill(const Plan& p, int dir, const pfft_spectral_op& op, int side, PassArgs& a);
 std::string plan_describe(const Plan& p);
 
 // device side (kernels.hip)
 int device_prepare(Plan& p);
 void device_release(Plan& p);
 int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
-                   hipStream_t stream);
+                   hipStream_t stream, const pfft_spectral_op* op = nullptr);
 int device_set_timing(Plan& p, bool on);
 int device_stage_times(Plan& p, int n, float* ms);
 int device_pass_names(Plan& p, int dir, std::string* out);

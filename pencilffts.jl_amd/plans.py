@@ -319,9 +319,11 @@ class PencilFFTPlan:
                 raise L.ArgumentError(L.ERR_ARGUMENT,
                                       "arrays must be contiguous GPU tensors")
 
-    def _execute(self, direction, src, dst, stream=None):
+    def _execute(self, direction, src, dst, stream=None, op=None):
+        """Run one direction (include/pfft.h PFFT_FORWARD ...); `op` is an
+        optional _lib.SpectralOp fused into the passes (pfft_execute_op)."""
         if self.inplace:
-            return self._execute_inplace(direction, src, dst, stream)
+            return self._execute_inplace(direction, src, dst, stream, op)
         # mul! and the transpose of bmul! read the input pencil; bmul!, ldiv!
         # and the transpose of mul! read the output pencil
         if direction in (L.FORWARD, L.ADJOINT_BACKWARD):
@@ -330,14 +332,21 @@ class PencilFFTPlan:
         else:
             self._check_arrays(src, dst, self.pencil_output, self.pencil_input,
                                self.eltype_output, self.eltype_input)
-        w = self._workspace()
         s = stream if stream is not None else torch.cuda.current_stream(src.parent.device)
-        L.check(L.lib.pfft_execute(self._h, direction, src.data_ptr(), dst.data_ptr(),
-                                   w.data_ptr() if w is not None else None,
-                                   self.work_bytes, s.cuda_stream))
+        self._launch(direction, src.data_ptr(), dst.data_ptr(), s, op)
         return dst
 
-    def _execute_inplace(self, direction, src, dst, stream=None):
+    def _launch(self, direction, pin, pout, s, op):
+        w = self._workspace()
+        wp = w.data_ptr() if w is not None else None
+        if op is None:
+            L.check(L.lib.pfft_execute(self._h, direction, pin, pout, wp, self.work_bytes,
+                                       s.cuda_stream))
+        else:
+            L.check(L.lib.pfft_execute_op(self._h, direction, C.byref(op), pin, pout, wp,
+                                          self.work_bytes, s.cuda_stream))
+
+    def _execute_inplace(self, direction, src, dst, stream=None, op=None):
         # operations.jl:100-116: the same ManyPencilArray in and out
         if not isinstance(src, ManyPencilArray) or not isinstance(dst, ManyPencilArray):
             raise L.ArgumentError(L.ERR_ARGUMENT, "array types (%s, %s) incompatible with "
@@ -351,11 +360,8 @@ class PencilFFTPlan:
             raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of output data")
         if src.nbytes() < self.in_bytes or not src.data.is_cuda:
             raise L.ArgumentError(L.ERR_ARGUMENT, "buffer too small for the in-place plan")
-        w = self._workspace()
         s = stream if stream is not None else torch.cuda.current_stream(src.data.device)
-        L.check(L.lib.pfft_execute(self._h, direction, src.data_ptr(), src.data_ptr(),
-                                   w.data_ptr() if w is not None else None,
-                                   self.work_bytes, s.cuda_stream))
+        self._launch(direction, src.data_ptr(), src.data_ptr(), s, op)
         return dst
 
     def _maybe_allocate(self, allocator, src):

@@ -28,6 +28,8 @@ class Emulator:
         L.pfft_emu_last_error.restype = C.c_char_p
         L.pfft_emu_nbytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.pfft_emu_execute.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.pfft_emu_execute_op.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
 
     def run(self, dims, kinds, real_type, direction, x, permute=True, extra=()):
         """x: numpy array holding the column-major memory image of the input
@@ -90,5 +92,31 @@ class Emulator:
             bi, bo = C.c_size_t(), C.c_size_t()
             self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
             return bi.value, bo.value
+        finally:
+            self.lib.pfft_emu_plan_destroy(h)
+
+    def run_op(self, dims, kinds, real_type, direction, x, op, out=None, permute=True, extra=()):
+        """pfft_emu_execute_op: `op` is a ctypes pfft_spectral_op; `out` (uint8,
+        optional) is the initial output for accumulate."""
+        N = len(dims)
+        h = C.c_void_p()
+        ne = len(extra)
+        st = self.lib.pfft_emu_plan_create(
+            C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
+            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), 0)
+        if st:
+            raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+        try:
+            bi, bo = C.c_size_t(), C.c_size_t()
+            self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
+            nin, nout = (bi.value, bo.value) if direction in (0, 4) else (bo.value, bi.value)
+            src = np.ascontiguousarray(x).reshape(-1)
+            assert src.nbytes == nin, (src.nbytes, nin)
+            res = np.zeros(nout, dtype=np.uint8) if out is None else out.copy()
+            st = self.lib.pfft_emu_execute_op(h, direction, C.addressof(op), src.ctypes.data,
+                                              res.ctypes.data)
+            if st:
+                raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+            return res
         finally:
             self.lib.pfft_emu_plan_destroy(h)

@@ -1,0 +1,132 @@
+"""Spectral operators fused into the passes on the GPU (pfft_execute_op
+through the C ABI), on shapes whose carrying passes run the specialised
+operator variants, against numpy multipliers on the FP64 oracle's spectra;
+transposes by the dot-product identity; the batched gradient workload built
+on them is covered in test_gpu_autodiff.py."""
+import numpy as np
+import pytest
+
+from oracle import pipeline as OP
+from tests.cases import CODE_NAMES, meta_for, random_logical, tol_for
+from tests.test_spectral_ops import multiplier
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _fill(a, x):
+    a.logical().copy_(torch.from_numpy(np.ascontiguousarray(x)).to(a.parent.dtype))
+
+
+def _image(a):
+    return a.parent.detach().cpu().numpy().reshape(-1)
+
+
+SHAPES = [(64, 32, 16), (128, 64, 32), (256, 128, 64), (96, 64, 48), (512, 64, 32),
+          (1024, 32, 16)]
+OPS = [(1, 0), (1, 1), (1, 2), (2, 0), (3, 0), (4, 0)]
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["F32", "F64"])
+@pytest.mark.parametrize("dims", SHAPES, ids=lambda d: "x".join(map(str, d)))
+def test_operators_forward_ldiv(pf, gpu, dims, real_type):
+    if real_type == 1 and max(dims) > 512:
+        pytest.skip("Float64 fast kernels stop at 512")
+    T = pf.Transforms
+    S = pf.spectral
+    meta, _ = meta_for(T, dims, (T.RFFT(),), CODE_NAMES[real_type])
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, real_type)
+    u = pf.allocate_input(plan)
+    v = pf.allocate_output(plan)
+    w = pf.allocate_input(plan)
+    rng = np.random.default_rng(4)
+    x = random_logical(rng, dims, "Float64")
+    _fill(u, x)
+    xd = u.logical().cpu().numpy().astype(np.float64)
+    y = OP.forward(meta, OP.to_memory(xd, meta.perm_in))
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    for op, dim in OPS:
+        mm = OP.to_memory(multiplier(op, dim, dims, True), meta.perm_out)
+        S.mul_(v, plan, u, op, dim=dim, alpha=0.5)
+        torch.cuda.synchronize()
+        err_f = OP.rel_l2(_image(v), OP.memory_image(0.5 * mm * y))
+        # ldiv with the operator from the exact spectrum
+        v.parent.copy_(torch.from_numpy(np.ascontiguousarray(np.transpose(y)))
+                       .to(v.parent.dtype))
+        S.ldiv_(w, plan, v, op, dim=dim, alpha=2.0)
+        torch.cuda.synchronize()
+        want = 2.0 * OP.backward(meta, mm * y, scaled=True)
+        err_b = OP.rel_l2(_image(w), OP.memory_image(np.real(want)))
+        print(dims, real_type, op, dim, "fwd %.2e ldiv %.2e" % (err_f, err_b))
+        assert err_f <= tol and err_b <= 10 * tol, (op, dim, err_f, err_b)
+
+
+@pytest.mark.parametrize("dims", [(64, 32, 16), (256, 128, 64), (512, 64, 32)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_operator_transposes_and_accumulate(pf, gpu, dims):
+    T = pf.Transforms
+    S = pf.spectral
+    L = pf._lib
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, T.Float64)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = pf.allocate_input(plan)
+    x.parent.normal_(generator=g)
+    yb = pf.allocate_output(plan)
+    yb.parent.copy_(torch.complex(torch.randn(yb.parent.shape, generator=g, device="cuda",
+                                              dtype=torch.float64),
+                                  torch.randn(yb.parent.shape, generator=g, device="cuda",
+                                              dtype=torch.float64)))
+    Ax = pf.allocate_output(plan)
+    ATy = pf.allocate_input(plan)
+    for op, dim in [(1, 0), (1, 2), (3, 0), (4, 0)]:
+        S.mul_(Ax, plan, x, op, dim=dim, alpha=1.5)
+        S.mul_adjoint_(ATy, plan, yb, op, dim=dim, alpha=1.5)
+        lhs = float(torch.sum((torch.conj(yb.parent) * Ax.parent).real))
+        rhs = float(torch.sum(ATy.parent * x.parent))
+        assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), 1.0), (op, dim, lhs, rhs)
+        By = pf.allocate_input(plan)
+        BTx = pf.allocate_output(plan)
+        S.bmul_(By, plan, yb, op, dim=dim)
+        S.bmul_adjoint_(BTx, plan, x, op, dim=dim)
+        lhs = float(torch.sum(x.parent * By.parent))
+        rhs = float(torch.sum((torch.conj(BTx.parent) * yb.parent).real))
+        assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), 1.0), (op, dim, lhs, rhs)
+    # accumulate: sum_j i k_j F(u_j) = three accumulating forwards
+    us = [pf.allocate_input(plan) for _ in range(3)]
+    for a in us:
+        a.parent.normal_(generator=g)
+    acc = pf.allocate_output(plan)
+    for j in range(3):
+        S.mul_(acc, plan, us[j], S.DERIVATIVE, dim=j, accumulate=j > 0)
+    ref = pf.allocate_output(plan)
+    tmp = pf.allocate_output(plan)
+    ref.parent.zero_()
+    for j in range(3):
+        S.mul_(tmp, plan, us[j], S.DERIVATIVE, dim=j)
+        ref.parent.add_(tmp.parent)
+    assert torch.allclose(acc.parent, ref.parent, rtol=1e-12, atol=1e-9)
+    assert L.OP_DERIVATIVE == S.DERIVATIVE
+
+
+def test_gradient_of_a_mode(pf, gpu):
+    """d/dx sin(3x) cos(2y) = 3 cos(3x) cos(2y), exactly, from the fused
+    derivative of the spectrum (the reference's gradient example,
+    docs/examples/gradient.jl)."""
+    T = pf.Transforms
+    S = pf.spectral
+    n = 64
+    plan = pf.PencilFFTPlan((n, n, n), T.RFFT(), (1,), None, T.Float64)
+    xs = torch.arange(n, dtype=torch.float64, device="cuda") * (2 * np.pi / n)
+    X, Y, Z = torch.meshgrid(xs, xs, xs, indexing="ij")
+    u = pf.allocate_input(plan)
+    u.logical().copy_(torch.sin(3 * X) * torch.cos(2 * Y))
+    uh = plan * u
+    d = pf.allocate_input(plan)
+    S.ldiv_(d, plan, uh, S.DERIVATIVE, dim=0)
+    assert torch.allclose(d.logical(), 3 * torch.cos(3 * X) * torch.cos(2 * Y), atol=1e-12)
+    S.ldiv_(d, plan, uh, S.DERIVATIVE, dim=1)
+    assert torch.allclose(d.logical(), -2 * torch.sin(3 * X) * torch.sin(2 * Y), atol=1e-12)
+    S.ldiv_(d, plan, uh, S.LAPLACIAN)
+    assert torch.allclose(d.logical(), -13 * torch.sin(3 * X) * torch.cos(2 * Y), atol=1e-10)
+    S.ldiv_(d, plan, uh, S.INV_LAPLACIAN)
+    assert torch.allclose(d.logical(), -torch.sin(3 * X) * torch.cos(2 * Y) / 13, atol=1e-13)
