@@ -633,6 +633,10 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     TileCtx c;
     tile_init(a, tl, c);
     C<R> v[LPW][NX];
+    // spectral operator: the state of the tile's shared line dims, once
+    [[maybe_unused]] OpLine<R> otile;
+    [[maybe_unused]] int64_t om0 = 0;
+    if constexpr (OPK) otile = op_tile<R>(a, tl, &om0);
 
     // ---------------- load and pre-process ----------------
     if constexpr (LD == IO_TILE) {
@@ -689,18 +693,20 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           v[q][p] = conj_in ? conj(z) : z;
         }
       } else if constexpr (LD == IO_TILE) {
-        OpLine ol;
-        if constexpr (OPK) {
-          int64_t m[kMaxDims];
-          line_index(a, tl, t < c.Tact ? t : 0, m);
-          ol = op_line(a, m);
-        }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          C<R> x = TK(p, t, 0);
-          if constexpr (OPK)
-            if (a.op_side == 1) x = x * op_factor<R>(a, ol, l + L * p);
+          const C<R> x = TK(p, t, 0);
           v[q][p] = conj_in ? conj(x) : x;
+        }
+        if constexpr (OPK) {
+          if (a.op_side == 1) {   // m x, applied before the conjugation of BFFT
+            const OpLine<R> ol = op_line<R>(a, otile, om0 + t);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              const C<R> f = op_factor<R>(a, ol, l + L * p);
+              v[q][p] = v[q][p] * (conj_in ? conj(f) : f);
+            }
+          }
         }
       } else {
         // contiguous line: complex elements, or real pairs (x[2m], x[2m+1]) for r2c
@@ -708,18 +714,20 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
                              ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
                              : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
         if (active) {
-          OpLine ol;
-          if constexpr (OPK) {
-            int64_t m[kMaxDims];
-            line_index(a, tl, t, m);
-            ol = op_line(a, m);
-          }
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            C<R> x = in[l + L * p];
-            if constexpr (OPK)
-              if (a.op_side == 1) x = x * op_factor<R>(a, ol, l + L * p);
+            const C<R> x = in[l + L * p];
             v[q][p] = conj_in ? conj(x) : x;
+          }
+          if constexpr (OPK) {
+            if (a.op_side == 1) {   // m x, applied before the conjugation of BFFT
+              const OpLine<R> ol = op_line<R>(a, otile, om0 + t);
+#pragma unroll
+              for (int p = 0; p < P; ++p) {
+                const C<R> f = op_factor<R>(a, ol, l + L * p);
+                v[q][p] = v[q][p] * (conj_in ? conj(f) : f);
+              }
+            }
           }
         }
       }
@@ -803,9 +811,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], scq);
         if constexpr (OPK) {
           if (a.op_side == 2) {
-            int64_t m[kMaxDims];
-            line_index(a, tl, active ? t : 0, m);
-            const OpLine ol = op_line(a, m);
+            const OpLine<R> ol = op_line<R>(a, otile, om0 + t);
 #pragma unroll
             for (int p = 0; p < P; ++p) v[q][p] = v[q][p] * op_factor<R>(a, ol, l + L * p);
           }
@@ -919,7 +925,11 @@ static void add_op(std::vector<Entry>& v) {
   constexpr int TL = 256 / Cfg<M>::L;
   constexpr int TT = sizeof(R) == 4 && M < 1024 ? 16 : 8;
   add<R, M, TL, 1, K_C2C, IO_LINE, IO_LINE, 0, 1, 1, 0, 1>(v);
+  add<R, M, TL * 2, 2, K_C2C, IO_LINE, IO_LINE, 0, 1, 1, 0, 1>(v);
   add<R, M, TT, 1, K_C2C, IO_LINE, IO_TILE, 0, 1, 1, 0, 1>(v);
+  add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE, 0, 1, 1, 0, 1>(v);
+  if constexpr (M >= 256 && (M & (M - 1)) == 0)
+    add<R, M, TT, 1, K_C2C, IO_LINE, IO_TILE, 0, 2, 1, 0, 1>(v);
   add<R, M, TT, 1, K_C2C, IO_TILE, IO_LINE, 0, 1, 1, 0, 1>(v);
 }
 

@@ -25,71 +25,76 @@ struct TileCtx {
   int32_t Tw;        // work lines (two per line for complex R2R data)
 };
 
-// Line multi-index (over the line dims) of line t of a tile.
-PFFT_HD void line_index(const PassArgs& a, int64_t tile, int t, int64_t* m) {
-  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
-  m[0] = (tile % nb0) * a.T + t;
-  int64_t rest = tile / nb0;
-  for (int d = 1; d < a.D; ++d) {
-    m[d] = rest % a.ldim[d];
-    rest /= a.ldim[d];
-  }
-}
-
 // Integer wavenumber of index c along logical dim d of the spectrum:
 // rfftfreq along the r2c dim, fftfreq elsewhere (test/include/Grids.jl:44-59
-// with a 2 pi box).
-PFFT_HD double wavenumber(const PassArgs& a, int d, int64_t c) {
-  const int64_t n = a.kn[d];
-  return a.kmode[d] == 1 ? (double)c : (double)(c < (n + 1) / 2 ? c : c - n);
+// with a 2 pi box).  Spectrum lengths stay below 2^31.
+template <typename R>
+PFFT_HD R wavenumber(const PassArgs& a, int d, int c) {
+  const int n = (int)a.kn[d];
+  return a.kmode[d] == 1 ? R(c) : R(c < (n + 1) / 2 ? c : c - n);
 }
 
-// Per-line part of a spectral operator: the wavenumbers of every logical dim
-// but the pass's own, folded into (A, B): m(k) = f(A, B, k_pass).
+// Wavenumber state of a spectral operator for one line: the contributions of
+// the logical dims other than the pass's own, folded into (a, b):
+//   derivative: a = k_dim (when dim is not the pass's);  |k|^2 forms: a = sum
+//   of k_d^2;  dealiasing: b = 0 when some |k_d| is past the 2/3 limit.
+template <typename R>
 struct OpLine {
-  double a, b;
+  R a, b;
 };
 
-PFFT_HD OpLine op_line(const PassArgs& a, const int64_t* m) {
-  double k[kMaxDims];
-  for (int d = 0; d < a.kN; ++d) k[d] = 0.0;
-  for (int d = 0; d < a.D; ++d)
-    if (a.llog[d] >= 0) k[a.llog[d]] = wavenumber(a, a.llog[d], m[d]);
-  OpLine o = {0.0, 1.0};
+template <typename R>
+PFFT_HD void op_add(const PassArgs& a, int d, int c, OpLine<R>& o) {
+  const int ld = a.llog[d];
+  if (ld < 0) return;
+  const R k = wavenumber<R>(a, ld, c);
   switch (a.op) {
-    case 1:   // i k_dim: the dim's wavenumber when it is not the pass's
-      o.a = a.op_dim == a.pdim ? 0.0 : k[a.op_dim];
-      break;
+    case 1: if (ld == a.op_dim) o.a = k; break;
     case 2:
-    case 3:   // |k|^2 over the other dims
-      for (int d = 0; d < a.kN; ++d)
-        if (d != a.pdim) o.a += k[d] * k[d];
-      break;
-    case 4:   // 2/3 rule over the other dims (navier_stokes.jl:331-344)
-      for (int d = 0; d < a.kN; ++d)
-        if (d != a.pdim && fabs(k[d]) > (2.0 / 3.0) * a.kmax[d]) o.b = 0.0;
-      break;
-    default:
-      break;
+    case 3: o.a += k * k; break;
+    case 4: if (fabs(k) > R(2.0 / 3.0) * R(a.kmax[ld])) o.b = R(0); break;
+    default: break;
   }
+}
+
+// The part shared by every line of a tile (line dims 1..D-1); *m0 receives
+// the index along line dim 0 of the tile's first line.
+template <typename R>
+PFFT_HD OpLine<R> op_tile(const PassArgs& a, int64_t tile, int64_t* m0) {
+  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
+  *m0 = (tile % nb0) * a.T;
+  int64_t rest = tile / nb0;
+  OpLine<R> o = {R(0), R(1)};
+  for (int d = 1; d < a.D; ++d) {
+    op_add<R>(a, d, (int)(rest % a.ldim[d]), o);
+    rest /= a.ldim[d];
+  }
+  return o;
+}
+
+// Line m0 + t of the tile: adds line dim 0.
+template <typename R>
+PFFT_HD OpLine<R> op_line(const PassArgs& a, OpLine<R> o, int64_t m) {
+  op_add<R>(a, 0, (int)m, o);
   return o;
 }
 
 // alpha m(k) (conjugated for the transposes) at position j along the pass dim.
 template <typename R>
-PFFT_HD cpx<R> op_factor(const PassArgs& a, const OpLine& o, int64_t j) {
-  const double kp = wavenumber(a, a.pdim, j);
-  double re = 0.0, im = 0.0;
+PFFT_HD cpx<R> op_factor(const PassArgs& a, const OpLine<R>& o, int j) {
+  const R kp = wavenumber<R>(a, a.pdim, j);
+  R re = R(0), im = R(0);
   switch (a.op) {
     case 1: im = a.op_dim == a.pdim ? kp : o.a; break;
     case 2: re = -(o.a + kp * kp); break;
-    case 3: { const double s = o.a + kp * kp; re = s > 0.0 ? -1.0 / s : 0.0; break; }
-    case 4: re = (o.b != 0.0 && fabs(kp) <= (2.0 / 3.0) * a.kmax[a.pdim]) ? 1.0 : 0.0; break;
-    default: re = 1.0; break;
+    case 3: { const R s2 = o.a + kp * kp; re = s2 > R(0) ? R(-1) / s2 : R(0); break; }
+    case 4:
+      re = (o.b != R(0) && fabs(kp) <= R(2.0 / 3.0) * R(a.kmax[a.pdim])) ? R(1) : R(0);
+      break;
+    default: re = R(1); break;
   }
-  re *= a.op_alpha;
-  im *= a.op_alpha;
-  return mk(R(re), R(a.op_conj ? -im : im));
+  const R al = (R)a.op_alpha;
+  return mk(re * al, (a.op_conj ? -im : im) * al);
 }
 
 PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
@@ -166,9 +171,9 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       case L_C2C:
         v = raw_cplx(a, raw, w, j);
         if (a.op && a.op_side == 1) {
-          int64_t m[kMaxDims];
-          line_index(a, c.tile, (int)w, m);
-          v = v * op_factor<R>(a, op_line(a, m), j);
+          int64_t m0;
+          const OpLine<R> ot = op_tile<R>(a, c.tile, &m0);
+          v = v * op_factor<R>(a, op_line<R>(a, ot, m0 + w), (int)j);
         }
         break;
       case L_R2C_HALF:
@@ -461,9 +466,9 @@ PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       else y = post_cplx(a, W, t, k);
       if (a.do_scale) y = scl(y, sc);
       if (a.op && a.op_side == 2) {
-        int64_t m[kMaxDims];
-        line_index(a, c.tile, (int)t, m);
-        y = y * op_factor<R>(a, op_line(a, m), k);
+        int64_t m0;
+        const OpLine<R> ot = op_tile<R>(a, c.tile, &m0);
+        y = y * op_factor<R>(a, op_line<R>(a, ot, m0 + t), (int)k);
         if (a.op_acc) y = y + ((const cpx<R>*)a.out)[g];
       }
       ((cpx<R>*)a.out)[g] = y;

@@ -80,13 +80,20 @@ static std::string pass_key(const Plan& p, const PassPlan& pp, const std::vector
   return o.str();
 }
 
+// op_side > 0 times an operator-carrying variant with a derivative operator
+// on that side (the arithmetic of the operator included).
 static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, void* out,
-                        hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
+                        hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms,
+                        int op_side = 0) {
   PassPlan pp = base;
   fast_apply(pp, variant);
   int r = fast_prepare(pp);
   if (r) return r;
   PassArgs a = pp.args;
+  if (op_side) {
+    const pfft_spectral_op op = {PFFT_OP_DERIVATIVE, 0, 1.0, 0};
+    op_fill(p, op_side == 2 ? PFFT_FORWARD : PFFT_BACKWARD, op, op_side, a);
+  }
   a.in = in;
   a.out = out;
   a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
@@ -112,34 +119,66 @@ static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, vo
 }
 
 static int autotune(Plan& p) {
-  struct Job { PassPlan* pp; std::vector<int> cands; std::string key; };
+  // A job times the candidates of one pass: a plain pass (op_side 0, the
+  // choice is applied to it) or the operator-carrying form of the last
+  // forward / first backward pass (op_side 2 / 1, the choice is kept aside
+  // in variant_op for pfft_execute_op).
+  struct Job {
+    PassPlan* pp;
+    PassPlan base;
+    std::vector<int> cands;
+    std::string key;
+    int op_side;
+  };
   std::vector<Job> jobs;
   size_t in_max = 0, out_max = 0;
+  const uint32_t forced = (p.flags >> 16) & 0xffu;
+  auto settle = [&](PassPlan& pp, int op_side, int v) {
+    if (op_side) pp.variant_op = v;
+    else fast_apply(pp, v);
+  };
+  auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side) {
+    std::vector<int> c = fast_candidates(p, base, op_side != 0);
+    if (c.empty()) return;
+    if (forced) {
+      settle(pp, op_side, c[(forced - 1) % c.size()]);
+      return;
+    }
+    const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "");
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      settle(pp, op_side, it->second);
+      return;
+    }
+    if (c.size() == 1 || (p.flags & PFFT_FLAG_NO_AUTOTUNE)) {
+      settle(pp, op_side, c[0]);
+      return;
+    }
+    jobs.push_back({&pp, base, c, key, op_side});
+    in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
+    out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
+  };
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (auto* list : {&p.fwd, &p.bwd}) {
-      for (PassPlan& pp : *list) {
-        std::vector<int> c = fast_candidates(p, pp);
-        if (c.empty()) continue;
-        const uint32_t forced = (p.flags >> 16) & 0xffu;
-        if (forced) {
-          fast_apply(pp, c[(forced - 1) % c.size()]);
-          continue;
-        }
-        const std::string key = pass_key(p, pp, c);
-        auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) {
-          fast_apply(pp, it->second);
-          continue;
-        }
-        if (c.size() == 1 || (p.flags & PFFT_FLAG_NO_AUTOTUNE)) {
-          fast_apply(pp, c[0]);
-          continue;
-        }
-        jobs.push_back({&pp, c, key});
-        in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
-        out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
+    for (auto* list : {&p.fwd, &p.bwd})
+      for (PassPlan& pp : *list) consider(pp, pp, 0);
+    // operator carriers (when the plan admits spectral operators)
+    const pfft_spectral_op probe = {PFFT_OP_DERIVATIVE, 0, 1.0, 0};
+    for (int dir : {PFFT_FORWARD, PFFT_BACKWARD}) {
+      int idx = -1, side = 0;
+      if (op_setup(p, dir, probe, &idx, &side) != PFFT_OK) {
+        clear_error();
+        continue;
       }
+      PassPlan& pp = dir == PFFT_FORWARD ? p.fwd[idx] : p.bwd[idx];
+      if (pp.variant == 0) continue;   // the generic pass carries it
+      PassPlan q = pp;
+      q.args = q.gen_args;
+      q.threads = q.gen_threads;
+      q.lds_bytes = q.gen_lds;
+      q.grid = q.gen_grid;
+      q.variant = 0;
+      consider(pp, q, side);
     }
   }
   if (jobs.empty()) return PFFT_OK;
@@ -160,7 +199,7 @@ static int autotune(Plan& p) {
       hipEventCreate(&e1) != hipSuccess) {
     cleanup();
     // no room to tune: fall back to the default variants
-    for (Job& j : jobs) fast_apply(*j.pp, j.cands[0]);
+    for (Job& j : jobs) settle(*j.pp, j.op_side, j.cands[0]);
     (void)hipGetLastError();
     return PFFT_OK;
   }
@@ -175,7 +214,7 @@ static int autotune(Plan& p) {
     for (int round = 0; round < 2 && !rc; ++round)
       for (size_t i = 0; i < j.cands.size(); ++i) {
         float ms = 0;
-        rc = time_variant(p, *j.pp, j.cands[i], in, out, st, e0, e1, &ms);
+        rc = time_variant(p, j.base, j.cands[i], in, out, st, e0, e1, &ms, j.op_side);
         if (rc) break;
         t[i] = ms < t[i] ? ms : t[i];
       }
@@ -184,7 +223,7 @@ static int autotune(Plan& p) {
     float best_ms = 1e30f;
     for (size_t i = 0; i < j.cands.size(); ++i)
       if (t[i] < best_ms) { best_ms = t[i]; best = j.cands[i]; }
-    fast_apply(*j.pp, best);
+    settle(*j.pp, j.op_side, best);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[j.key] = best;
   }
@@ -353,9 +392,13 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
       opp.grid = opp.gen_grid;
       opp.variant = 0;
       if (passes[i].variant != 0) {
-        const std::vector<int> c = fast_candidates(p, opp, true);
-        if (!c.empty()) {
-          fast_apply(opp, c[0]);
+        int v = passes[i].variant_op;
+        if (v < 0) {
+          const std::vector<int> c = fast_candidates(p, opp, true);
+          v = c.empty() ? 0 : c[0];
+        }
+        if (v > 0) {
+          fast_apply(opp, v);
           r = fast_prepare(opp);
           if (r) return r;
         }

@@ -31,6 +31,7 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 const char* last_error() { return g_last_error.c_str(); }
+void clear_error() { g_last_error.clear(); }
 
 // ---- transform metadata ------------------------------------------------------
 static const char* kR2RNames[11] = {"R2HC",    "HC2R",    "DHT",     "REDFT00",

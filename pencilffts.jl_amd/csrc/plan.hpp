@@ -22,6 +22,7 @@ struct Error {
 void set_error(int code, const std::string& msg);
 int fail(int code, const std::string& msg);  // set_error + return code
 const char* last_error();
+void clear_error();
 
 // ---- 1D transform metadata (src/Transforms) ---------------------------------
 bool kind_valid(int32_t kind, std::string* why);
@@ -54,6 +55,7 @@ struct PassPlan {
   PassArgs gen_args;
   int32_t gen_threads;
   int64_t gen_lds, gen_grid;
+  int32_t variant_op = -1;  // operator-carrying variant (-1: not tuned, 0: generic)
 };
 
 struct Table {

@@ -130,3 +130,41 @@ def test_gradient_of_a_mode(pf, gpu):
     assert torch.allclose(d.logical(), -13 * torch.sin(3 * X) * torch.cos(2 * Y), atol=1e-10)
     S.ldiv_(d, plan, uh, S.INV_LAPLACIAN)
     assert torch.allclose(d.logical(), -torch.sin(3 * X) * torch.cos(2 * Y) / 13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dims,real_type", [((256, 64, 32), 0), ((96, 64, 48), 0),
+                                            ((512, 32, 16), 1), ((1024, 32, 16), 0)],
+                         ids=["256F32", "96F32", "512F64", "1024F32"])
+def test_every_operator_variant(pf, gpu, dims, real_type):
+    """Each operator-carrying variant the tuner may pick (forced with
+    PFFT_FLAG_CANDIDATE) on both sides: mul! with the derivative and ldiv!
+    with the inverse Laplacian, against the oracle."""
+    T = pf.Transforms
+    S = pf.spectral
+    L = pf._lib
+    meta, _ = meta_for(T, dims, (T.RFFT(),), CODE_NAMES[real_type])
+    rng = np.random.default_rng(9)
+    x = random_logical(rng, dims, "Float64")
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    m1 = OP.to_memory(multiplier(1, 2, dims, True), meta.perm_out)
+    m3 = OP.to_memory(multiplier(3, 0, dims, True), meta.perm_out)
+    for k in range(6):
+        plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, real_type,
+                                flags=L.FLAG_CANDIDATE(k))
+        u = pf.allocate_input(plan)
+        v = pf.allocate_output(plan)
+        w = pf.allocate_input(plan)
+        _fill(u, x)
+        xd = u.logical().cpu().numpy().astype(np.float64)
+        y = OP.forward(meta, OP.to_memory(xd, meta.perm_in))
+        S.mul_(v, plan, u, S.DERIVATIVE, dim=2)
+        torch.cuda.synchronize()
+        err_f = OP.rel_l2(_image(v), OP.memory_image(m1 * y))
+        v.parent.copy_(torch.from_numpy(np.ascontiguousarray(np.transpose(y)))
+                       .to(v.parent.dtype))
+        S.ldiv_(w, plan, v, S.INV_LAPLACIAN)
+        torch.cuda.synchronize()
+        want = np.real(OP.backward(meta, m3 * y, scaled=True))
+        err_b = OP.rel_l2(_image(w), OP.memory_image(want))
+        print(k, dims, "fwd %.2e ldiv %.2e" % (err_f, err_b))
+        assert err_f <= tol and err_b <= 10 * tol, (k, err_f, err_b)
