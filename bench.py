@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--extra", action="append", default=None,
-                    choices=["plain", "precision_1024", "batched_graph"],
+                    choices=["plain", "small_64_f64", "precision_1024", "batched_graph"],
                     help="run only the named extra workload(s)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--flags", type=int, default=0)
@@ -89,6 +89,17 @@ def cpu_baseline(dims, transform, real, reps):
         OP.backward(meta, v, scaled=True, method="fast", workers=workers, single=single)
         times.append(time.perf_counter() - t0)
     ms = 1e3 * min(times)
+    # the reference's CPU-runnable case, forward and backward timed apart
+    m64 = OM.PlanMeta((64, 64, 64), ("RFFT",), "Float64")
+    x64 = np.asfortranarray(np.random.default_rng(42).uniform(0.0, 1.0, size=(64, 64, 64)))
+    tf, tb = [], []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        v64 = OP.forward(m64, x64, method="fast", workers=1)
+        t1 = time.perf_counter()
+        OP.backward(m64, v64, scaled=True, method="fast", workers=1)
+        tb.append(time.perf_counter() - t1)
+        tf.append(t1 - t0)
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -102,7 +113,10 @@ def cpu_baseline(dims, transform, real, reps):
             "sample": "%d full %s^3 %s %s pairs (oracle pipeline: permuted copy + scipy.fft per "
                       "stage, %s, workers=%d), best of %d; host CPU: %s"
                       % (reps, dims[0], real, transform, "float32" if single else "float64",
-                         workers, reps, cpu)}
+                         workers, reps, cpu),
+            "small_64_f64": {"ms_forward": round(1e3 * min(tf), 4),
+                             "ms_backward": round(1e3 * min(tb), 4), "cores": 1,
+                             "sample": "64^3 Float64 RFFT / ldiv!, best of 20"}}
 
 
 # ---------------------------------------------------------------- helpers
@@ -215,6 +229,45 @@ def extra_plain(torch, pf, steps, warmup):
     return res
 
 
+def extra_small_f64(torch, pf, steps, warmup):
+    """The reference's CPU-runnable case: 64^3 Float64 field, uniform [0,1)
+    with seed 42, RFFT then normalised BRFFT (ldiv!), forward and backward
+    timed separately, and the round-trip error."""
+    import numpy as np
+    T = pf.Transforms
+    dims = (64, 64, 64)
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float64)
+    u = pf.allocate_input(plan)
+    v = pf.allocate_output(plan)
+    w = pf.allocate_input(plan)
+    x = np.random.default_rng(42).uniform(0.0, 1.0, size=dims)
+    u.logical().copy_(torch.from_numpy(x).to("cuda"))
+    steps = max(steps, 100)
+    for _ in range(warmup):
+        pf.mul_(v, plan, u)
+        pf.ldiv_(w, plan, v)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(steps):
+        pf.mul_(v, plan, u)
+    ev[1].record(stream)
+    for _ in range(steps):
+        pf.ldiv_(w, plan, v)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1]) / steps
+    bwd = ev[1].elapsed_time(ev[2]) / steps
+    err = rel_l2_chunked(torch, w.logical().reshape(-1), u.logical().reshape(-1), 1)
+    out = {"workload": "64^3 Float64 RFFT + ldiv! round trip, uniform[0,1) seed 42",
+           "ms_forward": round(fwd, 5), "ms_backward": round(bwd, 5),
+           "round_trip_rel_l2": err, "kernels": plan.kernels(0) + plan.kernels(2)}
+    del plan, u, v, w
+    torch.cuda.empty_cache()
+    return out
+
+
 def extra_precision(torch, pf, steps, warmup, edge=1024):
     """1024^3 ComplexF32 FFT pair: plain FP32 radix stages vs radix-16 MFMA
     stages (split-FP16 hi+lo, BF16x3), error vs the Float64 plan."""
@@ -302,8 +355,8 @@ def extra_batched(torch, pf, steps, warmup):
 
 def run_extras(torch, pf, steps, warmup, only=None):
     res = {}
-    for key, fn in (("plain", extra_plain), ("precision_1024", extra_precision),
-                    ("batched_graph", extra_batched)):
+    for key, fn in (("plain", extra_plain), ("small_64_f64", extra_small_f64),
+                    ("precision_1024", extra_precision), ("batched_graph", extra_batched)):
         if only and key not in only:
             continue
         try:

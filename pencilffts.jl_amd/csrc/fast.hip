@@ -556,7 +556,9 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 // T / LPW (one per L lanes), so each line slot handles LPW lines in turn.
 // A transposed store goes out in NSP row chunks of M / NSP rows, so the
 // tile needs 1 / NSP of the rows in LDS and more workgroups fit on a CU.
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP>
+// PF: the prefetching variants keep the twiddles (and the c2r merge factors)
+// in LDS after the tile / line slots.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int PF = 0>
 constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per line
   constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
   constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
@@ -564,8 +566,50 @@ constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per li
   constexpr size_t tld = LD == IO_TILE ? (size_t)rows_ld * T * sizeof(C<R>) : 0;
   constexpr size_t tst = ST == IO_TILE ? (size_t)rows_st * T * sizeof(C<R>) : 0;
   constexpr size_t tile = tld > tst ? tld : tst;
-  return lines > tile ? lines : tile;
+  constexpr size_t base = lines > tile ? lines : tile;
+  constexpr size_t tables =
+      PF ? (size_t)(M + (KIND == K_C2R || KIND == K_R2C ? M + 1 : 0)) * sizeof(C<R>) : 0;
+  return base + tables;
 }
+
+// Register staging of a transposed input tile (prefetching variants): the
+// loads of the next tile are issued before the current tile is transformed
+// and land in LDS at the top of the next round.
+template <typename R, int T, int ROWS, int NTHR>
+struct TileStage {
+  static constexpr int EPV = sizeof(R) == 4 ? 2 : 1;   // complex per 16-byte vector
+  static constexpr int VPR = T / EPV;
+  static constexpr int TOTAL = ROWS * VPR;
+  static constexpr int FULL = TOTAL / NTHR;
+  static constexpr int N = (TOTAL + NTHR - 1) / NTHR;
+  // a native vector type: HIP's float4 struct is copied by memcpy, which
+  // keeps the array in scratch memory
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  v4 r[N];
+  __device__ __forceinline__ void fetch(const PassArgs& a, int64_t base, int tid) {
+    const C<R>* in = (const C<R>*)a.in;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int e = tid + i * NTHR;
+      if (i < FULL || e < TOTAL) {
+        const int k = e / VPR;
+        const int t = EPV * (e - k * VPR);
+        r[i] = *(const v4*)&in[base + (int64_t)k * a.sk_in + t];
+      }
+    }
+  }
+  __device__ __forceinline__ void put(C<R>* tile, int tid) const {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int e = tid + i * NTHR;
+      if (i < FULL || e < TOTAL) {
+        const int k = e / VPR;
+        const int t = EPV * (e - k * VPR);
+        *(v4*)&tile[tile_idx<R, T>(k, t)] = r[i];
+      }
+    }
+  }
+};
 
 // OCC > 0 asks the compiler for at least OCC waves per SIMD (a register
 // budget of 512 / OCC): more waves in flight at the price of possible spills;
@@ -573,8 +617,12 @@ constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per li
 // OPK = 1 compiles in the spectral operator of pfft_execute_op (c2c only):
 // the elements read (op_side 1) or written (op_side 2) are multiplied by
 // alpha m(k); those variants run only the passes that carry an operator.
+// PF = 1: a persistent grid where each workgroup issues the loads of its
+// next tile (into registers: the transposed tile, or its lines) before
+// transforming the current one, with the twiddles and split/merge factors
+// read from LDS so that no other global load waits behind the prefetch.
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
-          int OCC, int OPK>
+          int OCC, int OPK, int PF>
 __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L))
 __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
@@ -619,8 +667,21 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     if constexpr (ADDT) return tile[tile_idx<R, T>(L - l, t) + (M - L * (p + 1)) * T];
     else return tile[tile_idx<R, T>(M - l - L * p, t)];
   };
-  const C<R>* __restrict__ tw = (const C<R>*)a.tw;
-  const C<R>* __restrict__ aux = (const C<R>*)a.aux;
+  static_assert(PF == 0 || (MF == 0 && W == 1 && OPK == 0 &&
+                            (LD == IO_TILE ? KIND == K_C2C || KIND == K_C2R
+                                           : KIND == K_C2C || KIND == K_R2C)),
+                "prefetching variants: c2c, c2r with transposed loads, r2c with line loads");
+  constexpr bool PFT = PF && LD == IO_TILE, PFL = PF && LD == IO_LINE;
+  constexpr size_t TBL = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
+  if constexpr (PF) {
+    C<R>* st = (C<R>*)(smem + TBL);
+    for (int i = tid; i < M; i += NTHR) st[i] = ((const C<R>*)a.tw)[i];
+    if constexpr (KIND == K_C2R || KIND == K_R2C)
+      for (int i = tid; i <= M; i += NTHR) st[M + i] = ((const C<R>*)a.aux)[i];
+    if constexpr (PFL) __syncthreads();   // line loads reach the FFT without a barrier
+  }
+  const C<R>* __restrict__ tw = PF ? (const C<R>*)(smem + TBL) : (const C<R>*)a.tw;
+  const C<R>* __restrict__ aux = PF ? (const C<R>*)(smem + TBL) + M : (const C<R>*)a.aux;
   // backward FFTs run as conj(FFT(conj x))
   const bool conj_in = KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C);
   const bool conj_out = KIND == K_C2C && a.bwd != 0;
@@ -629,9 +690,36 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   MfmaA<MF == 0 ? 1 : MF> mfA;
   if constexpr (MF != 0) mfA.init(l % 64);
 
+  [[maybe_unused]] TileStage<R, T, PFT ? ROWS_IN : 1, NTHR> stg;
+  // the next tile's lines (contiguous loads), as native vectors
+  typedef R cv2 __attribute__((ext_vector_type(2)));
+  [[maybe_unused]] cv2 nl[PFL ? LPW : 1][PFL ? P : 1];
+  auto fetch_lines = [&](const TileCtx& cc) {
+    if constexpr (PFL) {
+#pragma unroll
+      for (int q = 0; q < LPW; ++q) {
+        const int t = q * TQ + g;
+        const cv2* in = (const cv2*)((const R*)a.in +
+                                     (KIND == K_R2C ? 1 : 2) * (cc.in_base + (int64_t)t * a.lsin[0]));
+        if (t < cc.Tact) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) nl[q][p] = in[l + L * p];
+        }
+      }
+    }
+  };
+  [[maybe_unused]] TileCtx cn;
+  if constexpr (PF) {
+    if (blockIdx.x < a.ntiles) {
+      tile_init(a, blockIdx.x, cn);
+      if constexpr (PFT) stg.fetch(a, cn.in_base, tid);
+      else fetch_lines(cn);
+    }
+  }
   for (int64_t tl = blockIdx.x; tl < a.ntiles; tl += gridDim.x) {
     TileCtx c;
-    tile_init(a, tl, c);
+    if constexpr (PF) c = cn;
+    else tile_init(a, tl, c);
     C<R> v[LPW][NX];
     // spectral operator: the state of the tile's shared line dims, once
     [[maybe_unused]] OpLine<R> otile;
@@ -639,7 +727,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     if constexpr (OPK) otile = op_tile<R>(a, tl, &om0);
 
     // ---------------- load and pre-process ----------------
-    if constexpr (LD == IO_TILE) {
+    if constexpr (PFT) {
+      // the previous round ended with a barrier: the tile is free (and, on
+      // the first round, the tables are written)
+      stg.put(tile, tid);
+      __syncthreads();
+      const int64_t nx = tl + gridDim.x;
+      if (nx < a.ntiles) {
+        tile_init(a, nx, cn);
+        stg.fetch(a, cn.in_base, tid);
+      }
+    } else if constexpr (LD == IO_TILE) {
       tile_load<R, T, ROWS_IN, NTHR>(tile, a, c.in_base, tid);
       __syncthreads();
     }
@@ -716,7 +814,9 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         if (active) {
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const C<R> x = in[l + L * p];
+            C<R> x;
+            if constexpr (PFL) x = mk(nl[q][p].x, nl[q][p].y);
+            else x = in[l + L * p];
             v[q][p] = conj_in ? conj(x) : x;
           }
           if constexpr (OPK) {
@@ -730,6 +830,13 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
             }
           }
         }
+      }
+    }
+    if constexpr (PFL) {
+      const int64_t nx = tl + gridDim.x;
+      if (nx < a.ntiles) {
+        tile_init(a, nx, cn);
+        fetch_lines(cn);
       }
     }
     if constexpr (LD == IO_TILE) __syncthreads();   // line slots alias the tile
@@ -865,20 +972,21 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
 
 // ---- registry --------------------------------------------------------------------
 struct Entry {
-  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w, occ, opk;
+  int real_bytes, M, kind, ld, st, T, lpw, threads, mf, nsp, w, occ, opk, pf;
   size_t lds;
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const PassArgs&);
 };
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
-          int OCC, int OPK>
+          int OCC, int OPK, int PF>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>), g, b, lds, s, a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>), g, b, lds, s,
+                     a);
 }
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1,
-          int W = 1, int OCC = 0, int OPK = 0>
+          int W = 1, int OCC = 0, int OPK = 0, int PF = 0>
 static void add(std::vector<Entry>& v) {
   Entry e;
   e.real_bytes = sizeof(R);
@@ -893,10 +1001,11 @@ static void add(std::vector<Entry>& v) {
   e.w = W;
   e.occ = OCC;
   e.opk = OPK;
+  e.pf = PF;
   e.threads = T / LPW * Cfg<M, W>::L;
-  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>;
-  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK>;
+  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, PF>();
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>;
+  e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>;
   v.push_back(e);
 }
 
@@ -966,12 +1075,19 @@ static void add_size(std::vector<Entry>& v) {
     add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE>(v);
     add<R, M, TT, 2, K_C2R, IO_TILE, IO_LINE>(v);
     add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE>(v);
+    // transposed loads with the next tile prefetched (persistent grid)
+    add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE, 0, 1, 1, 0, 0, 1>(v);
     if constexpr (sizeof(R) == 4 && (M == 256 || M == 512)) {
       // register budgets for more waves in flight on the r2c / c2r of 512 and
       // 1024 reals (small spills, kept only where the tuner finds them faster)
       add<R, M, TT, 1, K_C2R, IO_TILE, IO_LINE, 0, 1, 1, 6>(v);
       add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE, 0, 1, 1, 8>(v);
     }
+    // line loads of the next tile prefetched (persistent grid); on c2c lines
+    // the loop-top wait also drains the previous tile's stores, which costs
+    // more than the prefetch gains, so only the r2c form is kept
+    if constexpr (M >= 64 && (M & (M - 1)) == 0)
+      add<R, M, TT, 1, K_R2C, IO_LINE, IO_TILE, 0, 1, 1, 0, 0, 1>(v);
     if constexpr (M >= 256 && (M & (M - 1)) == 0) {
       // row-chunked transposed stores: half (quarter) of the tile in LDS
       add<R, M, TT, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
@@ -1139,6 +1255,17 @@ void fast_apply(PassPlan& pp, int variant) {
   for (int d = 1; d < a.D; ++d) nt *= a.ldim[d];
   a.ntiles = nt;
   pp.grid = std::min<int64_t>(nt, 1 << 20);
+  if (e.pf) {
+    // persistent grid: the workgroups resident at once on the device
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.fn, e.threads, e.lds) ==
+            hipSuccess &&
+        cus > 0 && per_cu > 0)
+      pp.grid = std::min<int64_t>(nt, (int64_t)cus * per_cu);
+    (void)hipGetLastError();
+  }
   pp.threads = e.threads;
   pp.lds_bytes = (int64_t)e.lds;
   pp.variant = variant;
@@ -1177,7 +1304,7 @@ const char* fast_name(int variant) {
       std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
       (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
-      (e.opk ? ",op" : "") +
+      (e.opk ? ",op" : "") + (e.pf ? ",PF" : "") +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

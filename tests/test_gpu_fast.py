@@ -93,3 +93,51 @@ def test_mixed_workload_full_size(pf, gpu):
     print("mixed 384x384x256: fwd %.2e rt %.2e" % (err_f, err_b))
     tol = 5e-7 * np.log2(float(np.prod(dims)))
     assert err_f <= tol and err_b <= tol
+
+
+def test_cpu_case_64_f64_round_trip(pf, gpu):
+    """The reference's CPU-runnable case (64^3 Float64, uniform [0,1) with
+    seed 42, RFFT then ldiv!): spectrum against numpy's rfftn, round trip."""
+    T = pf.Transforms
+    dims = (64, 64, 64)
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float64)
+    x = np.random.default_rng(42).uniform(0.0, 1.0, size=dims)
+    u = pf.allocate_input(plan)
+    u.logical().copy_(torch.from_numpy(x).to("cuda"))
+    v = plan * u
+    # numpy's rfftn halves the last axis; the plan halves logical axis 0
+    ref = np.fft.rfftn(x.transpose(2, 1, 0)).transpose(2, 1, 0)
+    got = v.logical().cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-14 * np.linalg.norm(ref) * np.log2(x.size)
+    w = plan.ldiv(v)
+    assert np.max(np.abs(w.logical().cpu().numpy() - x)) <= 1e-14
+
+
+@pytest.mark.parametrize("dims,real_type,permute", [
+    ((256, 512, 128), 0, True), ((512, 128, 96), 0, True), ((128, 256, 64), 1, True),
+    ((128, 512, 32), 1, True)],
+    ids=["256x512x128-F32", "512x128x96-F32", "128x256x64-F64", "128x512x32-F64"])
+def test_prefetching_tile_loads(pf, gpu, dims, real_type, permute):
+    """The prefetching transposed-load c2r kernels (persistent grid, several
+    tiles per workgroup, the next tile's loads in flight), forced one at a
+    time, against the oracle."""
+    T = pf.Transforms
+    L = pf._lib
+    tr = T.RFFT()
+    seen = {}
+    for k in range(16):
+        plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, real_type, permute_dims=permute,
+                                flags=L.FLAG_CANDIDATE(k))
+        for name in plan.kernels(0) + plan.kernels(2):
+            if ",PF" in name and name not in seen:
+                seen[name] = k
+        del plan
+    assert seen, "no prefetching kernel among the candidates"
+    n = int(np.prod(dims))
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
+    for name, k in seen.items():
+        err_f, err_b, meta = run_case(pf, dims, (tr,), real_type, permute=permute, seed=k,
+                                      flags=L.FLAG_CANDIDATE(k))
+        print(name, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+        assert name in meta.kernels
+        assert err_f <= tol and err_b <= tol, (name, err_f, err_b)
