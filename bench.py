@@ -13,7 +13,12 @@ K pairs on the launch stream):
   * the precision form: 1024^3 ComplexF32 FFT pair with plain FP32 radix
     stages, split-FP16 MFMA and BF16x3 MFMA radix-16 stages, each with its
     relative L2 error against the Float64 plan of the same input;
-  * 12 uneven Float32 vector fields, loss + gradient in one hipGraph replay.
+  * 12 uneven Float32 vector fields, loss + gradient in one hipGraph replay;
+  * the 64^3 Float64 round trip (forward and backward timed apart).
+
+The headline line also carries per-pair mean/std/min/max, the error of the
+forward transform against the Float64 plan, a measured device-copy ceiling
+and the compulsory (fused 3D) bytes.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -45,6 +50,9 @@ def parse():
     ap.add_argument("--extra", action="append", default=None,
                     choices=["plain", "small_64_f64", "precision_1024", "batched_graph"],
                     help="run only the named extra workload(s)")
+    ap.add_argument("--no-checks", action="store_true",
+                    help="skip the Float64-plan error and the copy-ceiling measurement "
+                         "(profiled runs: only the pair's own kernels)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -89,6 +97,17 @@ def cpu_baseline(dims, transform, real, reps):
         OP.backward(meta, v, scaled=True, method="fast", workers=workers, single=single)
         times.append(time.perf_counter() - t0)
     ms = 1e3 * min(times)
+    # the same pair as one fused n-dimensional scipy.fft call per direction
+    # (no pencil chain): a second CPU reference point
+    import scipy.fft as sfft
+    tfused = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        if transform == "RFFT":
+            sfft.irfftn(sfft.rfftn(x, workers=workers), s=dims, workers=workers)
+        else:
+            sfft.ifftn(sfft.fftn(x, workers=workers), workers=workers)
+        tfused.append(time.perf_counter() - t0)
     # the reference's CPU-runnable case, forward and backward timed apart
     m64 = OM.PlanMeta((64, 64, 64), ("RFFT",), "Float64")
     x64 = np.asfortranarray(np.random.default_rng(42).uniform(0.0, 1.0, size=(64, 64, 64)))
@@ -114,6 +133,7 @@ def cpu_baseline(dims, transform, real, reps):
                       "stage, %s, workers=%d), best of %d; host CPU: %s"
                       % (reps, dims[0], real, transform, "float32" if single else "float64",
                          workers, reps, cpu),
+            "fused_nd_ms_per_pair": round(1e3 * min(tfused), 3),
             "small_64_f64": {"ms_forward": round(1e3 * min(tf), 4),
                              "ms_backward": round(1e3 * min(tb), 4), "cores": 1,
                              "sample": "64^3 Float64 RFFT / ldiv!, best of 20"}}
@@ -165,6 +185,64 @@ def time_pairs(torch, pf, plan, u, v, steps, warmup):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def pair_stats(torch, pf, plan, u, v, n):
+    """mean / std / min / max ms of n pairs timed one by one (HIP events on
+    the launch stream), as the reference's benchmark reports them."""
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+    ev[0].record(stream)
+    for i in range(n):
+        pf.mul_(v, plan, u)
+        pf.ldiv_(u, plan, v)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    t = [ev[i].elapsed_time(ev[i + 1]) for i in range(n)]
+    mean = sum(t) / n
+    std = math.sqrt(sum((x - mean) ** 2 for x in t) / max(1, n - 1))
+    return {"n": n, "mean": round(mean, 4), "std": round(std, 4), "min": round(min(t), 4),
+            "max": round(max(t), 4)}
+
+
+def copy_ceiling(torch, nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes per second) of a 1 GiB
+    buffer: the measured ceiling beside the datasheet peak."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def error_vs_f64(torch, pf, dims, tr, u32):
+    """Relative L2 error of the Float32 forward transform against the Float64
+    plan on the same (rounded) input."""
+    T = pf.Transforms
+    p32 = pf.PencilFFTPlan(dims, tr, (1, 1), None, T.Float32)
+    v32 = pf.allocate_output(p32)
+    pf.mul_(v32, p32, u32)
+    p64 = pf.PencilFFTPlan(dims, tr, (1, 1), None, T.Float64)
+    u64 = pf.allocate_input(p64)
+    u64.parent.copy_(u32.parent)
+    v64 = pf.allocate_output(p64)
+    pf.mul_(v64, p64, u64)
+    torch.cuda.synchronize()
+    err = rel_l2_chunked(torch, v32.parent, v64.parent)
+    del p32, p64, u64, v64, v32
+    torch.cuda.empty_cache()
+    return err
 
 
 def pass_times(pf, plan, u, v, nrep):
@@ -400,6 +478,7 @@ def main():
     # per-pass kernel durations from HIP events recorded by the library on the
     # launch stream (pfft_plan_set_timing), averaged over a few pairs
     fwd_ms, bwd_ms = pass_times(pf, plan, u, v, max(3, min(args.steps, 20)))
+    stats = pair_stats(torch, pf, plan, u, v, max(5, min(args.steps, 100)))
     fwd_bytes = pass_bytes(pf, plan)
     bwd_bytes = list(reversed(fwd_bytes))
     X = dims[0]
@@ -438,12 +517,26 @@ def main():
                    "parallelism": "single GPU"},
         "pair_hbm_frac": round(pair_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "pair_algorithmic_bytes": pair_bytes,
+        "pair_ms_stats": stats,
         "pairs_per_s": round(1e3 / ms, 2),
         "wall_s": round(wall, 3),
         "roofline": roofline,
     }
+    # compulsory bytes of a fused 3D transform (read the input, write the
+    # output once per direction): the lower bound beside the 3-pass figure
+    comp = 2 * (u.parent.numel() * u.parent.element_size() +
+                v.parent.numel() * v.parent.element_size())
+    out["compulsory_bytes"] = comp
+    out["compulsory_hbm_frac"] = round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if real_type == T.Float32 and not args.flags and not args.no_checks:
+        u.parent.uniform_(-1, 1, generator=torch.Generator(device="cuda").manual_seed(42))
+        out["rel_l2_forward_vs_f64_plan"] = float("%.3e" % error_vs_f64(torch, pf, dims, tr, u))
     del plan, u, v
     torch.cuda.empty_cache()
+    if not args.no_checks:
+        cc = copy_ceiling(torch)
+        out["copy_ceiling_GBs"] = round(cc, 1)
+        out["pair_frac_of_copy_ceiling"] = round(pair_bytes / (ms * 1e-3) / 1e9 / cc, 4)
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
