@@ -1178,6 +1178,11 @@ static const std::vector<Entry>& registry() {
 
 using namespace fk;
 
+// Variant ids: registry index + 1, with kOrderBit set for the alternative
+// tile order (tile_split order 1) of a transposing kernel.
+constexpr int kOrderBit = 1 << 20;
+static int variant_index(int variant) { return (variant & (kOrderBit - 1)) - 1; }
+
 static bool all_even(const int64_t* s, int from, int D) {
   for (int d = from; d < D; ++d)
     if (s[d] & 1) return false;
@@ -1228,6 +1233,11 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
       const int64_t sk = ld == IO_TILE ? a.sk_in : a.sk_out;
       const int64_t* ls = ld == IO_TILE ? a.lsin : a.lsout;
       if (rb == 4 && ((sk & 1) || !all_even(ls, 1, a.D))) continue;
+      // both tile orders: a far-strided transposed side favours walking the
+      // other line dims first
+      out.push_back((int)i + 1);
+      if (a.D > 1) out.push_back(((int)i + 1) | kOrderBit);
+      continue;
     }
     out.push_back((int)i + 1);
   }
@@ -1235,8 +1245,9 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
 }
 
 void fast_apply(PassPlan& pp, int variant) {
-  const Entry& e = registry()[variant - 1];
+  const Entry& e = registry()[variant_index(variant)];
   PassArgs& a = pp.args;
+  a.tile_order = (variant & kOrderBit) ? 1 : 0;
   // contiguous passes: keep the input-adjacent line dim first (its partial
   // last tile costs little) unless it is shorter than a tile
   if (e.ld == IO_LINE && e.st == IO_LINE && a.ldim[0] < e.T) {
@@ -1271,8 +1282,8 @@ void fast_apply(PassPlan& pp, int variant) {
   pp.variant = variant;
 }
 
-int fast_prepare(PassPlan& pp) {
-  const Entry& e = registry()[pp.variant - 1];
+int fast_prepare(const PassPlan& pp) {
+  const Entry& e = registry()[variant_index(pp.variant)];
   if (e.lds > 64 * 1024) {
     hipError_t r = hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)e.lds);
@@ -1284,10 +1295,11 @@ int fast_prepare(PassPlan& pp) {
 
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream) {
   const std::vector<Entry>& reg = registry();
-  if (pp.variant < 1 || pp.variant > (int)reg.size())
+  const int idx = variant_index(pp.variant);
+  if (pp.variant < 1 || idx < 0 || idx >= (int)reg.size())
     return fail(PFFT_ERR_UNSUPPORTED, "no specialised kernel variant " +
                                           std::to_string(pp.variant));
-  const Entry& e = reg[pp.variant - 1];
+  const Entry& e = reg[idx];
   e.launch(dim3((unsigned)pp.grid), dim3(e.threads), e.lds, stream, a);
   return PFFT_OK;
 }
@@ -1295,8 +1307,9 @@ int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream) {
 const char* fast_name(int variant) {
   static thread_local std::string s;
   const std::vector<Entry>& reg = registry();
-  if (variant < 1 || variant > (int)reg.size()) return "generic";
-  const Entry& e = reg[variant - 1];
+  const int idx = variant_index(variant);
+  if (variant < 1 || idx < 0 || idx >= (int)reg.size()) return "generic";
+  const Entry& e = reg[idx];
   static const char* kinds[5] = {"c2c", "r2c", "c2r", "dct2c", "dct3c"};
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
       std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
@@ -1305,6 +1318,7 @@ const char* fast_name(int variant) {
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
       (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
       (e.opk ? ",op" : "") + (e.pf ? ",PF" : "") +
+      ((variant & kOrderBit) ? ",ord1" : "") +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

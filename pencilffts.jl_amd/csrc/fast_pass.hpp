@@ -10,7 +10,7 @@ namespace pfft {
 std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op = false);
 // Switches a pass to a specialised variant (tile shape, grid, LDS).
 void fast_apply(PassPlan& pp, int variant);
-int fast_prepare(PassPlan& pp);
+int fast_prepare(const PassPlan& pp);
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream);
 const char* fast_name(int variant);  // kernel name of a variant ("generic" for 0)
 

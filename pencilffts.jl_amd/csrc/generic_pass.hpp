@@ -25,6 +25,23 @@ struct TileCtx {
   int32_t Tw;        // work lines (two per line for complex R2R data)
 };
 
+// Tile index -> (block b0 of T lines along line dim 0, linear index `rest`
+// over line dims 1..D-1).  Order 0 walks line dim 0 first (consecutive tiles
+// are neighbouring blocks of one run); order 1 walks the other dims first, so
+// that workgroups running at the same time touch addresses spread along
+// those dims rather than one far-strided run.
+PFFT_HD void tile_split(const PassArgs& a, int64_t tile, int64_t* b0, int64_t* rest) {
+  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
+  if (a.tile_order == 0) {
+    *b0 = tile % nb0;
+    *rest = tile / nb0;
+  } else {
+    const int64_t nrest = a.ntiles / nb0;
+    *b0 = tile / nrest;
+    *rest = tile - *b0 * nrest;
+  }
+}
+
 // Integer wavenumber of index c along logical dim d of the spectrum:
 // rfftfreq along the r2c dim, fftfreq elsewhere (test/include/Grids.jl:44-59
 // with a 2 pi box).  Spectrum lengths stay below 2^31.
@@ -61,9 +78,9 @@ PFFT_HD void op_add(const PassArgs& a, int d, int c, OpLine<R>& o) {
 // the index along line dim 0 of the tile's first line.
 template <typename R>
 PFFT_HD OpLine<R> op_tile(const PassArgs& a, int64_t tile, int64_t* m0) {
-  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
-  *m0 = (tile % nb0) * a.T;
-  int64_t rest = tile / nb0;
+  int64_t b0, rest;
+  tile_split(a, tile, &b0, &rest);
+  *m0 = b0 * a.T;
   OpLine<R> o = {R(0), R(1)};
   for (int d = 1; d < a.D; ++d) {
     op_add<R>(a, d, (int)(rest % a.ldim[d]), o);
@@ -98,9 +115,8 @@ PFFT_HD cpx<R> op_factor(const PassArgs& a, const OpLine<R>& o, int j) {
 }
 
 PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
-  const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
-  const int64_t b0 = tile % nb0;
-  int64_t rest = tile / nb0;
+  int64_t b0, rest;
+  tile_split(a, tile, &b0, &rest);
   int64_t ib = b0 * a.T * a.lsin[0];
   int64_t ob = b0 * a.T * a.lsout[0];
   for (int d = 1; d < a.D; ++d) {

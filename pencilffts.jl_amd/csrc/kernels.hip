@@ -81,10 +81,13 @@ static std::string pass_key(const Plan& p, const PassPlan& pp, const std::vector
 }
 
 // op_side > 0 times an operator-carrying variant with a derivative operator
-// on that side (the arithmetic of the operator included).
+// on that side (the arithmetic of the operator included).  With a producer
+// (the pass before this one in the chain, already tuned), each timed launch
+// follows a launch of the producer writing this pass's input, so the input
+// meets the caches as it does in the pipeline.
 static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, void* out,
-                        hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms,
-                        int op_side = 0) {
+                        const PassPlan* prod, void* prod_in, hipStream_t st, hipEvent_t e0,
+                        hipEvent_t e1, float* ms, int op_side = 0) {
   PassPlan pp = base;
   fast_apply(pp, variant);
   int r = fast_prepare(pp);
@@ -98,13 +101,25 @@ static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, vo
   a.out = out;
   a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
   a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
-  r = fast_launch(pp, a, st);  // warm-up
+  PassArgs pa;
+  if (prod) {
+    pa = prod->args;
+    pa.in = prod_in;
+    pa.out = in;
+    pa.tw = prod->tw_id >= 0 ? p.tables[prod->tw_id].dev : nullptr;
+    pa.aux = prod->aux_id >= 0 ? p.tables[prod->aux_id].dev : nullptr;
+  }
+  auto produce = [&]() { return prod ? fast_launch(*prod, pa, st) : PFFT_OK; };
+  r = produce();
+  if (!r) r = fast_launch(pp, a, st);  // warm-up
   if (r) return r;
-  // the median of a few single launches: a typical launch, robust to the
-  // occasional slow one (a minimum would favour erratic variants)
-  constexpr int trials = 5;
-  float t[trials];
+  // the median of a few launches: a typical launch, robust to the occasional
+  // slow one (a minimum would favour erratic variants); fewer on big passes
+  const int trials = (double)pp.in_elems * eltype_bytes(pp.elt_in) > 4e9 ? 3 : 5;
+  float t[5];
   for (int i = 0; i < trials; ++i) {
+    r = produce();
+    if (r) return r;
     HIP_TRY(hipEventRecord(e0, st));
     r = fast_launch(pp, a, st);
     if (r) return r;
@@ -129,15 +144,16 @@ static int autotune(Plan& p) {
     std::vector<int> cands;
     std::string key;
     int op_side;
+    const PassPlan* prod;   // the pass writing this one's input (nullptr: none)
   };
   std::vector<Job> jobs;
-  size_t in_max = 0, out_max = 0;
+  size_t in_max = 0, out_max = 0, prod_max = 0;
   const uint32_t forced = (p.flags >> 16) & 0xffu;
   auto settle = [&](PassPlan& pp, int op_side, int v) {
     if (op_side) pp.variant_op = v;
     else fast_apply(pp, v);
   };
-  auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side) {
+  auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side, const PassPlan* prod) {
     std::vector<int> c = fast_candidates(p, base, op_side != 0);
     if (c.empty()) return;
     if (forced) {
@@ -154,14 +170,19 @@ static int autotune(Plan& p) {
       settle(pp, op_side, c[0]);
       return;
     }
-    jobs.push_back({&pp, base, c, key, op_side});
+    jobs.push_back({&pp, base, c, key, op_side, prod});
     in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
     out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
+    if (prod) {
+      prod_max = std::max(prod_max, (size_t)prod->in_elems * eltype_bytes(prod->elt_in));
+      in_max = std::max(in_max, (size_t)prod->out_elems * eltype_bytes(prod->elt_out));
+    }
   };
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto* list : {&p.fwd, &p.bwd})
-      for (PassPlan& pp : *list) consider(pp, pp, 0);
+      for (size_t i = 0; i < list->size(); ++i)
+        consider((*list)[i], (*list)[i], 0, i > 0 ? &(*list)[i - 1] : nullptr);
     // operator carriers (when the plan admits spectral operators)
     const pfft_spectral_op probe = {PFFT_OP_DERIVATIVE, 0, 1.0, 0};
     for (int dir : {PFFT_FORWARD, PFFT_BACKWARD}) {
@@ -170,7 +191,8 @@ static int autotune(Plan& p) {
         clear_error();
         continue;
       }
-      PassPlan& pp = dir == PFFT_FORWARD ? p.fwd[idx] : p.bwd[idx];
+      std::vector<PassPlan>& list = dir == PFFT_FORWARD ? p.fwd : p.bwd;
+      PassPlan& pp = list[idx];
       if (pp.variant == 0) continue;   // the generic pass carries it
       PassPlan q = pp;
       q.args = q.gen_args;
@@ -178,11 +200,11 @@ static int autotune(Plan& p) {
       q.lds_bytes = q.gen_lds;
       q.grid = q.gen_grid;
       q.variant = 0;
-      consider(pp, q, side);
+      consider(pp, q, side, idx > 0 ? &list[idx - 1] : nullptr);
     }
   }
   if (jobs.empty()) return PFFT_OK;
-  void *in = nullptr, *out = nullptr;
+  void *in = nullptr, *out = nullptr, *pin = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PFFT_OK;
@@ -190,11 +212,13 @@ static int autotune(Plan& p) {
     if (st) (void)hipStreamSynchronize(st);
     if (in) (void)hipFree(in);
     if (out) (void)hipFree(out);
+    if (pin) (void)hipFree(pin);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
   };
   if (hipMalloc(&in, in_max) != hipSuccess || hipMalloc(&out, out_max) != hipSuccess ||
+      (prod_max && hipMalloc(&pin, prod_max) != hipSuccess) ||
       hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
     cleanup();
@@ -206,7 +230,13 @@ static int autotune(Plan& p) {
   // a nonzero input pattern (1.0f words): the kernels' time does not depend
   // on the data, but the memory system is exercised as with real fields
   (void)hipMemsetD32Async((hipDeviceptr_t)in, 0x3f800000u, in_max / 4, st);
+  if (pin) (void)hipMemsetD32Async((hipDeviceptr_t)pin, 0x3f800000u, prod_max / 4, st);
   for (Job& j : jobs) {
+    // the producer, as tuned (a generic producer is left out: its LDS limit
+    // is set after tuning)
+    const PassPlan* prod = j.prod && j.prod->variant != 0 ? j.prod : nullptr;
+    if (prod) rc = fast_prepare(*prod);
+    if (rc) break;
     // two rounds over all candidates, keeping each one's better median: the
     // first round also brings the clocks up, so early candidates are not
     // judged cold
@@ -214,7 +244,8 @@ static int autotune(Plan& p) {
     for (int round = 0; round < 2 && !rc; ++round)
       for (size_t i = 0; i < j.cands.size(); ++i) {
         float ms = 0;
-        rc = time_variant(p, j.base, j.cands[i], in, out, st, e0, e1, &ms, j.op_side);
+        rc = time_variant(p, j.base, j.cands[i], in, out, prod, pin, st, e0, e1, &ms,
+                          j.op_side);
         if (rc) break;
         t[i] = ms < t[i] ? ms : t[i];
       }

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-MAX_CANDIDATES = 16
+MAX_CANDIDATES = 24
 
 # (dims, transforms, real type, precision flags): each shape puts its passes
 # on line->line, line->tile and tile->line kernels of one length family
