@@ -471,19 +471,56 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 3, 2)
 enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4 };
 enum { IO_LINE = 0, IO_TILE = 1 };
 
-// tile element (row k, line t): XOR swizzle keeps each row's 16-byte pairs
-// adjacent (float) while spreading the column writes over the banks
-template <typename R, int T>
+// tile element (row k, line t) sits at column t ^ tile_xor(k) of row k.
+// LDS has 64 banks of 4 bytes; a 64-bit access is served 32 lanes per cycle,
+// a 128-bit one 16 lanes per cycle.
+// SW = 1 (lines of 32 or more lanes (float) / 16 or more (double)): the XOR
+// is (k / RPC) mod T, RPC = rows per 256 bytes, so a column written or read by
+// 32 (float) / 16 (double) consecutive rows meets 32 / 16 distinct bank slots.
+// A float pair (t, t+1), t even, stays in one aligned 16-byte slot, in swapped
+// order when the XOR is odd (see tile_vec).
+// SW = 0: the XOR keeps float pairs in order ((k mod T/2) * 2, (k mod T)
+// for double): 2- to 4-way column conflicts, but it repeats every T/2 (T) rows,
+// which lines of fewer lanes need for lane-base addressing.
+template <typename R, int T, int SW>
+__device__ __forceinline__ int tile_xor(int k) {
+  if constexpr (SW) {
+    constexpr int RPC = (sizeof(R) == 4 ? 32 : 16) / T;
+    static_assert(RPC >= 1, "tile row wider than 256 bytes");
+    return (k / RPC) & (T - 1);
+  } else if constexpr (sizeof(R) == 4) {
+    return (k & (T / 2 - 1)) << 1;
+  } else {
+    return k & (T - 1);
+  }
+}
+template <typename R, int T, int SW>
 __device__ __forceinline__ int tile_idx(int k, int t) {
-  if constexpr (sizeof(R) == 4) return k * T + (t ^ ((k & (T / 2 - 1)) << 1));
-  else return k * T + (t ^ (k & (T - 1)));
+  return k * T + (t ^ tile_xor<R, T, SW>(k));
+}
+// 16-byte vector of row k holding columns (t, t + EPV - 1), t a multiple of
+// EPV: its element offset, and whether its two float elements are swapped.
+template <typename R, int T, int SW>
+__device__ __forceinline__ int tile_vec(int k, int t, bool* swap) {
+  const int x = tile_xor<R, T, SW>(k);
+  if constexpr (sizeof(R) == 4) {
+    *swap = (x & 1) != 0;
+    return k * T + ((t ^ x) & ~1);
+  } else {
+    *swap = false;
+    return k * T + (t ^ x);
+  }
+}
+typedef float tile_v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ tile_v4 swap_pair(tile_v4 v, bool sw) {
+  return sw ? tile_v4{v.z, v.w, v.x, v.y} : v;
 }
 
 // Cooperative tile copies.  Full rounds (every thread busy) are unrolled
 // with compile-time trip counts, so a thread issues all its global loads
 // before the first LDS write; the partial last round is a separate guarded
 // step that keeps nothing live across its branch.
-template <typename R, int T, int ROWS, int NTHR, bool ACC = false>
+template <typename R, int T, int ROWS, int NTHR, int SW, bool ACC = false>
 __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, int64_t base,
                                            int tid) {
   C<R>* out = (C<R>*)a.out;
@@ -506,26 +543,30 @@ __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, 
   constexpr int VPR = T / EPV;                     // vectors per row
   constexpr int TOTAL = ROWS * VPR;
   constexpr int FULL = TOTAL / NTHR;
+  auto get = [&](int k, int t) -> float4 {
+    bool sw;
+    const int o = tile_vec<R, T, SW>(k, t, &sw);
+    const tile_v4 v = swap_pair(*(const tile_v4*)&tile[o], sw);
+    return make_float4(v.x, v.y, v.z, v.w);
+  };
 #pragma unroll
   for (int i = 0; i < FULL; ++i) {
     const int e = tid + i * NTHR;
     const int k = e / VPR;
     const int t = EPV * (e - k * VPR);
-    put((float4*)&out[base + (int64_t)k * a.sk_out + t],
-        *(const float4*)&tile[tile_idx<R, T>(k, t)]);
+    put((float4*)&out[base + (int64_t)k * a.sk_out + t], get(k, t));
   }
   if constexpr (TOTAL % NTHR != 0) {
     const int e = tid + FULL * NTHR;
     if (e < TOTAL) {
       const int k = e / VPR;
       const int t = EPV * (e - k * VPR);
-      put((float4*)&out[base + (int64_t)k * a.sk_out + t],
-          *(const float4*)&tile[tile_idx<R, T>(k, t)]);
+      put((float4*)&out[base + (int64_t)k * a.sk_out + t], get(k, t));
     }
   }
 }
 
-template <typename R, int T, int ROWS, int NTHR>
+template <typename R, int T, int ROWS, int NTHR, int SW>
 __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t base, int tid) {
   const C<R>* in = (const C<R>*)a.in;
   constexpr int EPV = sizeof(R) == 4 ? 2 : 1;
@@ -534,20 +575,24 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
   constexpr int FULL = TOTAL / NTHR;
   // global and LDS accesses never alias, so the unrolled loads are hoisted
   // ahead of the LDS writes
+  auto set = [&](int k, int t, tile_v4 v) {
+    bool sw;
+    const int o = tile_vec<R, T, SW>(k, t, &sw);
+    *(tile_v4*)&tile[o] = swap_pair(v, sw);
+  };
 #pragma unroll
   for (int i = 0; i < FULL; ++i) {
     const int e = tid + i * NTHR;
     const int k = e / VPR;
     const int t = EPV * (e - k * VPR);
-    *(float4*)&tile[tile_idx<R, T>(k, t)] = *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
+    set(k, t, *(const tile_v4*)&in[base + (int64_t)k * a.sk_in + t]);
   }
   if constexpr (TOTAL % NTHR != 0) {
     const int e = tid + FULL * NTHR;
     if (e < TOTAL) {
       const int k = e / VPR;
       const int t = EPV * (e - k * VPR);
-      *(float4*)&tile[tile_idx<R, T>(k, t)] =
-          *(const float4*)&in[base + (int64_t)k * a.sk_in + t];
+      set(k, t, *(const tile_v4*)&in[base + (int64_t)k * a.sk_in + t]);
     }
   }
 }
@@ -575,7 +620,7 @@ constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per li
 // Register staging of a transposed input tile (prefetching variants): the
 // loads of the next tile are issued before the current tile is transformed
 // and land in LDS at the top of the next round.
-template <typename R, int T, int ROWS, int NTHR>
+template <typename R, int T, int ROWS, int NTHR, int SW>
 struct TileStage {
   static constexpr int EPV = sizeof(R) == 4 ? 2 : 1;   // complex per 16-byte vector
   static constexpr int VPR = T / EPV;
@@ -605,7 +650,9 @@ struct TileStage {
       if (i < FULL || e < TOTAL) {
         const int k = e / VPR;
         const int t = EPV * (e - k * VPR);
-        *(v4*)&tile[tile_idx<R, T>(k, t)] = r[i];
+        bool sw;
+        const int o = tile_vec<R, T, SW>(k, t, &sw);
+        *(v4*)&tile[o] = swap_pair(r[i], sw);
       }
     }
   }
@@ -647,7 +694,9 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   // Lane bases for the points k = l + L p and M - k = (M - L (p+1)) + (L - l):
   // pad8 is additive over multiples of 8 and the tile swizzle repeats every
   // SWP rows, so with L a multiple of those each access is base + constant.
-  constexpr int SWP = sizeof(R) == 4 ? T / 2 : T;
+  // conflict-free tile swizzle when the line's lanes cover its period
+  constexpr int SW = L % (sizeof(R) == 4 ? 32 : 16) == 0 && T * sizeof(C<R>) <= 256 ? 1 : 0;
+  constexpr int SWP = SW ? (sizeof(R) == 4 ? 32 : 16) : (sizeof(R) == 4 ? T / 2 : T);
   constexpr bool ADD8 = L % 8 == 0, ADDT = L % SWP == 0;
   C<R>* const lp = line + pad8(l);
   C<R>* const lm = line + pad8(L - l);
@@ -660,12 +709,12 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     else return line[pad8(M - l - L * p)];
   };
   auto TK = [&](int p, int t, int k0) -> C<R>& {   // tile[tile_idx(k0 + l + L p, t)]
-    if constexpr (ADDT) return tile[tile_idx<R, T>(l, t) + (k0 + L * p) * T];
-    else return tile[tile_idx<R, T>(k0 + l + L * p, t)];
+    if constexpr (ADDT) return tile[tile_idx<R, T, SW>(l, t) + (k0 + L * p) * T];
+    else return tile[tile_idx<R, T, SW>(k0 + l + L * p, t)];
   };
   auto TMK = [&](int p, int t) -> C<R>& {          // tile[tile_idx(M - l - L p, t)]
-    if constexpr (ADDT) return tile[tile_idx<R, T>(L - l, t) + (M - L * (p + 1)) * T];
-    else return tile[tile_idx<R, T>(M - l - L * p, t)];
+    if constexpr (ADDT) return tile[tile_idx<R, T, SW>(L - l, t) + (M - L * (p + 1)) * T];
+    else return tile[tile_idx<R, T, SW>(M - l - L * p, t)];
   };
   static_assert(PF == 0 || (MF == 0 && W == 1 && OPK == 0 &&
                             (LD == IO_TILE ? KIND == K_C2C || KIND == K_C2R
@@ -690,7 +739,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   MfmaA<MF == 0 ? 1 : MF> mfA;
   if constexpr (MF != 0) mfA.init(l % 64);
 
-  [[maybe_unused]] TileStage<R, T, PFT ? ROWS_IN : 1, NTHR> stg;
+  [[maybe_unused]] TileStage<R, T, PFT ? ROWS_IN : 1, NTHR, SW> stg;
   // the next tile's lines (contiguous loads), as native vectors
   typedef R cv2 __attribute__((ext_vector_type(2)));
   [[maybe_unused]] cv2 nl[PFL ? LPW : 1][PFL ? P : 1];
@@ -738,7 +787,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         stg.fetch(a, cn.in_base, tid);
       }
     } else if constexpr (LD == IO_TILE) {
-      tile_load<R, T, ROWS_IN, NTHR>(tile, a, c.in_base, tid);
+      tile_load<R, T, ROWS_IN, NTHR, SW>(tile, a, c.in_base, tid);
       __syncthreads();
     }
 #pragma unroll
@@ -755,7 +804,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           line_sync<L>();
         }
         auto X = [&](int k) -> C<R> {
-          if constexpr (LD == IO_TILE) return tile[tile_idx<R, T>(k, t)];
+          if constexpr (LD == IO_TILE) return tile[tile_idx<R, T, SW>(k, t)];
           else return line[pad8(k)];
         };
         auto XK = [&](int p) -> C<R> {    // x_k, k = l + L p
@@ -952,17 +1001,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
 #pragma unroll
           for (int p = h * PCH; p < (h + 1) * PCH; ++p) TK(p, t, -h * RCH) = v[q][p];
           if constexpr (KIND == K_R2C)
-            if (h == NSP - 1 && l == 0) tile[tile_idx<R, T>(M - h * RCH, t)] = v[q][P];
+            if (h == NSP - 1 && l == 0) tile[tile_idx<R, T, SW>(M - h * RCH, t)] = v[q][P];
         }
         __syncthreads();
         const int64_t ob = c.out_base + (int64_t)(h * RCH) * a.sk_out;
         constexpr int LAST_ROWS = ROWS_OUT - (NSP - 1) * RCH;
         if (OPK && a.op_acc) {
-          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, true>(tile, a, ob, tid);
-          else tile_store<R, T, RCH, NTHR, true>(tile, a, ob, tid);
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW, true>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR, SW, true>(tile, a, ob, tid);
         } else {
-          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR>(tile, a, ob, tid);
-          else tile_store<R, T, RCH, NTHR>(tile, a, ob, tid);
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR, SW>(tile, a, ob, tid);
         }
       }
     }
