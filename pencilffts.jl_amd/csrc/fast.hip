@@ -204,7 +204,9 @@ template <> struct Cfg<1024> {
   }
 };
 // two wavefronts per line: half the registers per lane, stages synchronised
-// by workgroup barriers
+// by workgroup barriers.  (Twice the lanes on 128- and 256-point lines, at 4
+// points per lane with radix-4 stages, measured 5-25 % slower than W = 1:
+// one more LDS round trip per line outweighs the extra waves.)
 template <> struct Cfg<1024, 2> {
   static constexpr int L = 128, P = 8;
   template <typename R>
@@ -716,7 +718,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     if constexpr (ADDT) return tile[tile_idx<R, T, SW>(L - l, t) + (M - L * (p + 1)) * T];
     else return tile[tile_idx<R, T, SW>(M - l - L * p, t)];
   };
-  static_assert(PF == 0 || (MF == 0 && W == 1 && OPK == 0 &&
+  static_assert(PF == 0 || (MF == 0 && L <= 64 && OPK == 0 &&
                             (LD == IO_TILE ? KIND == K_C2C || KIND == K_C2R
                                            : KIND == K_C2C || KIND == K_R2C)),
                 "prefetching variants: c2c, c2r with transposed loads, r2c with line loads");

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <map>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -85,9 +87,11 @@ static std::string pass_key(const Plan& p, const PassPlan& pp, const std::vector
 // (the pass before this one in the chain, already tuned), each timed launch
 // follows a launch of the producer writing this pass's input, so the input
 // meets the caches as it does in the pipeline.
+static constexpr int kTrials = 5;
 static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, void* out,
-                        const PassPlan* prod, void* prod_in, hipStream_t st, hipEvent_t e0,
-                        hipEvent_t e1, float* ms, int op_side = 0) {
+                        const PassPlan* prod, void* prod_in, hipStream_t st,
+                        const hipEvent_t* e0, const hipEvent_t* e1, float* ms,
+                        int op_side = 0) {
   PassPlan pp = base;
   fast_apply(pp, variant);
   int r = fast_prepare(pp);
@@ -114,38 +118,188 @@ static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, vo
   if (!r) r = fast_launch(pp, a, st);  // warm-up
   if (r) return r;
   // the median of a few launches: a typical launch, robust to the occasional
-  // slow one (a minimum would favour erratic variants); fewer on big passes
-  const int trials = (double)pp.in_elems * eltype_bytes(pp.elt_in) > 4e9 ? 3 : 5;
-  float t[5];
+  // slow one (a minimum would favour erratic variants); fewer on big passes.
+  // The launches are queued back to back and waited for once, so the device
+  // stays busy as in the pipeline.
+  const int trials = (double)pp.in_elems * eltype_bytes(pp.elt_in) > 4e9 ? 3 : kTrials;
+  float t[kTrials];
   for (int i = 0; i < trials; ++i) {
     r = produce();
     if (r) return r;
-    HIP_TRY(hipEventRecord(e0, st));
+    HIP_TRY(hipEventRecord(e0[i], st));
     r = fast_launch(pp, a, st);
     if (r) return r;
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
-    HIP_TRY(hipEventElapsedTime(&t[i], e0, e1));
+    HIP_TRY(hipEventRecord(e1[i], st));
   }
+  HIP_TRY(hipEventSynchronize(e1[trials - 1]));
+  for (int i = 0; i < trials; ++i) HIP_TRY(hipEventElapsedTime(&t[i], e0[i], e1[i]));
   HIP_TRY(hipGetLastError());
   std::sort(t, t + trials);
   *ms = t[trials / 2];
   return PFFT_OK;
 }
 
-static int autotune(Plan& p) {
-  // A job times the candidates of one pass: a plain pass (op_side 0, the
-  // choice is applied to it) or the operator-carrying form of the last
-  // forward / first backward pass (op_side 2 / 1, the choice is kept aside
-  // in variant_op for pfft_execute_op).
-  struct Job {
-    PassPlan* pp;
-    PassPlan base;
-    std::vector<int> cands;
-    std::string key;
-    int op_side;
-    const PassPlan* prod;   // the pass writing this one's input (nullptr: none)
+// A job times the candidates of one pass: a plain pass (op_side 0, the
+// choice is applied to it) or the operator-carrying form of the last forward /
+// first backward pass (op_side 2 / 1, the choice is kept aside in variant_op
+// for pfft_execute_op).
+struct TuneJob {
+  PassPlan* pp;
+  PassPlan base;
+  std::vector<int> cands;
+  std::string key;
+  int op_side;
+  const PassPlan* prod;   // the pass writing this one's input (nullptr: none)
+};
+
+// One pass of the chain on the given buffers (the timing loops of the tuner).
+static int launch_pass(const Plan& p, const PassPlan& pp, const void* src, void* dst,
+                       bool scale, hipStream_t st) {
+  PassArgs a = pp.args;
+  a.in = src;
+  a.out = dst;
+  a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
+  a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+  if (scale) {
+    a.do_scale = 1;
+    a.scale = 1.0 / p.scale;
+  }
+  if (pp.variant == 0) {
+    if (p.real_type == PFFT_F32)
+      hipLaunchKernelGGL(k_generic<float>, dim3((unsigned)pp.grid), dim3(pp.threads),
+                         (size_t)pp.lds_bytes, st, a);
+    else
+      hipLaunchKernelGGL(k_generic<double>, dim3((unsigned)pp.grid), dim3(pp.threads),
+                         (size_t)pp.lds_bytes, st, a);
+    return PFFT_OK;
+  }
+  return fast_launch(pp, a, st);
+}
+
+static int prepare_pass(const Plan& p, const PassPlan& pp) {
+  if (pp.variant != 0) return fast_prepare(pp);
+  return p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
+                                 : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);
+}
+
+// Second tuning phase: every plain pass's candidates again, each timed inside
+// whole mul! + ldiv! pairs on buffers of the plan's sizes (the other passes
+// at their current choice), so the caches, clocks and neighbouring kernels
+// are those of the pipeline.  Out-of-place plans; skipped when the buffers
+// do not fit.
+static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
+  if (p.inplace) return PFFT_OK;
+  bool any = false;
+  for (const TuneJob& j : jobs) any = any || j.op_side == 0;
+  if (!any) return PFFT_OK;
+  void *u = nullptr, *v = nullptr, *w = nullptr;
+  hipStream_t st = nullptr;
+  const int kPairs = 3;
+  hipEvent_t e0[kPairs] = {}, e1[kPairs] = {};
+  auto cleanup = [&]() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* b : {u, v, w})
+      if (b) (void)hipFree(b);
+    for (int i = 0; i < kPairs; ++i) {
+      if (e0[i]) (void)hipEventDestroy(e0[i]);
+      if (e1[i]) (void)hipEventDestroy(e1[i]);
+    }
+    if (st) (void)hipStreamDestroy(st);
   };
+  bool ok = hipMalloc(&u, p.in_bytes) == hipSuccess && hipMalloc(&v, p.out_bytes) == hipSuccess &&
+            (!p.work_bytes || hipMalloc(&w, p.work_bytes) == hipSuccess) &&
+            hipStreamCreate(&st) == hipSuccess;
+  for (int i = 0; ok && i < kPairs; ++i)
+    ok = hipEventCreate(&e0[i]) == hipSuccess && hipEventCreate(&e1[i]) == hipSuccess;
+  if (!ok) {   // no room: keep the first phase's choices
+    cleanup();
+    (void)hipGetLastError();
+    return PFFT_OK;
+  }
+  (void)hipMemsetD32Async((hipDeviceptr_t)u, 0x3f800000u, p.in_bytes / 4, st);
+  void* bufs[3] = {nullptr, w, w ? (char*)w + p.work0_bytes : nullptr};
+  // runs `pairs` mul! + ldiv! pairs, recording events around pass `idx` of
+  // list `lst` (0 fwd, 1 bwd) in pair i
+  auto run_pairs = [&](int lst, int idx, int pairs) -> int {
+    for (int i = 0; i < pairs; ++i) {
+      for (int d = 0; d < 2; ++d) {
+        const std::vector<PassPlan>& passes = d == 0 ? p.fwd : p.bwd;
+        void* in = d == 0 ? u : v;
+        void* out = d == 0 ? v : u;
+        const int np = (int)passes.size();
+        for (int k = 0; k < np; ++k) {
+          const PassPlan& pp = passes[k];
+          const void* src = pp.src == 0 ? in : bufs[pp.src];
+          void* dst = pp.dst == 0 ? out : bufs[pp.dst];
+          const bool timed = d == lst && k == idx && i < kPairs;
+          if (timed) HIP_TRY(hipEventRecord(e0[i], st));
+          const int r = launch_pass(p, pp, src, dst, d == 1 && k == np - 1, st);
+          if (r) return r;
+          if (timed) HIP_TRY(hipEventRecord(e1[i], st));
+        }
+      }
+    }
+    HIP_TRY(hipGetLastError());
+    return PFFT_OK;
+  };
+  int rc = PFFT_OK;
+  for (auto* list : {&p.fwd, &p.bwd})
+    for (PassPlan& pp : *list) {
+      rc = prepare_pass(p, pp);
+      if (rc) break;
+    }
+  const double pair_bytes = 2.0 * ((double)p.in_bytes + (double)p.out_bytes);
+  const int rounds = pair_bytes > 8e9 ? 1 : 2;
+  for (TuneJob& j : jobs) {
+    if (rc) break;
+    if (j.op_side) continue;
+    const int lst = j.pp >= p.fwd.data() && j.pp < p.fwd.data() + p.fwd.size() ? 0 : 1;
+    const int idx = (int)(j.pp - (lst == 0 ? p.fwd.data() : p.bwd.data()));
+    const int vop = j.pp->variant_op;
+    auto apply = [&](int var) {
+      *j.pp = j.base;
+      fast_apply(*j.pp, var);
+      j.pp->variant_op = vop;
+      return fast_prepare(*j.pp);
+    };
+    std::vector<float> t(j.cands.size(), 1e30f);
+    for (int round = 0; round < rounds && !rc; ++round)
+      for (size_t c = 0; c < j.cands.size() && !rc; ++c) {
+        rc = apply(j.cands[c]);
+        if (!rc) rc = run_pairs(lst, idx, 1);   // warm-up
+        if (!rc) rc = run_pairs(lst, idx, kPairs);
+        if (rc) break;
+        if (hipEventSynchronize(e1[kPairs - 1]) != hipSuccess) {
+          rc = fail(PFFT_ERR_DEVICE, "autotune: event synchronisation failed");
+          break;
+        }
+        float ts[kPairs];
+        for (int i = 0; i < kPairs; ++i)
+          if (hipEventElapsedTime(&ts[i], e0[i], e1[i]) != hipSuccess) ts[i] = 1e30f;
+        std::sort(ts, ts + kPairs);
+        t[c] = std::min(t[c], ts[kPairs / 2]);
+      }
+    if (rc) break;
+    size_t best = 0;
+    for (size_t c = 1; c < j.cands.size(); ++c)
+      if (t[c] < t[best]) best = c;
+    if (log) {
+      fprintf(stderr, "pfft autotune (in chain): %s pass %d\n", lst == 0 ? "forward" : "backward",
+              idx + 1);
+      for (size_t c = 0; c < j.cands.size(); ++c)
+        fprintf(stderr, "  %9.4f ms %s%s\n", t[c], fast_name(j.cands[c]),
+                c == best ? "  <- chosen" : "");
+    }
+    rc = apply(j.cands[best]);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[j.key] = j.cands[best];
+  }
+  cleanup();
+  return rc;
+}
+
+static int autotune(Plan& p) {
+  using Job = TuneJob;
   std::vector<Job> jobs;
   size_t in_max = 0, out_max = 0, prod_max = 0;
   const uint32_t forced = (p.flags >> 16) & 0xffu;
@@ -206,21 +360,27 @@ static int autotune(Plan& p) {
   if (jobs.empty()) return PFFT_OK;
   void *in = nullptr, *out = nullptr, *pin = nullptr;
   hipStream_t st = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e0[kTrials] = {}, e1[kTrials] = {};
   int rc = PFFT_OK;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
     if (in) (void)hipFree(in);
     if (out) (void)hipFree(out);
     if (pin) (void)hipFree(pin);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
+    for (int i = 0; i < kTrials; ++i) {
+      if (e0[i]) (void)hipEventDestroy(e0[i]);
+      if (e1[i]) (void)hipEventDestroy(e1[i]);
+    }
     if (st) (void)hipStreamDestroy(st);
   };
   if (hipMalloc(&in, in_max) != hipSuccess || hipMalloc(&out, out_max) != hipSuccess ||
       (prod_max && hipMalloc(&pin, prod_max) != hipSuccess) ||
-      hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-      hipEventCreate(&e1) != hipSuccess) {
+      hipStreamCreate(&st) != hipSuccess || [&]() {
+        for (int i = 0; i < kTrials; ++i)
+          if (hipEventCreate(&e0[i]) != hipSuccess || hipEventCreate(&e1[i]) != hipSuccess)
+            return true;
+        return false;
+      }()) {
     cleanup();
     // no room to tune: fall back to the default variants
     for (Job& j : jobs) settle(*j.pp, j.op_side, j.cands[0]);
@@ -231,34 +391,50 @@ static int autotune(Plan& p) {
   // on the data, but the memory system is exercised as with real fields
   (void)hipMemsetD32Async((hipDeviceptr_t)in, 0x3f800000u, in_max / 4, st);
   if (pin) (void)hipMemsetD32Async((hipDeviceptr_t)pin, 0x3f800000u, prod_max / 4, st);
-  for (Job& j : jobs) {
+  // PFFT_TUNE_LOG=1 in the environment prints every timing and choice
+  const char* log_env = getenv("PFFT_TUNE_LOG");
+  const bool tune_log = log_env && log_env[0] == '1';
+  auto tune = [&](Job& j, const PassPlan* prod) -> int {
     // the producer, as tuned (a generic producer is left out: its LDS limit
     // is set after tuning)
-    const PassPlan* prod = j.prod && j.prod->variant != 0 ? j.prod : nullptr;
-    if (prod) rc = fast_prepare(*prod);
-    if (rc) break;
+    if (prod && prod->variant == 0) prod = nullptr;
+    if (prod) {
+      const int r = fast_prepare(*prod);
+      if (r) return r;
+    }
     // two rounds over all candidates, keeping each one's better median: the
     // first round also brings the clocks up, so early candidates are not
     // judged cold
     std::vector<float> t(j.cands.size(), 1e30f);
-    for (int round = 0; round < 2 && !rc; ++round)
+    for (int round = 0; round < 2; ++round)
       for (size_t i = 0; i < j.cands.size(); ++i) {
         float ms = 0;
-        rc = time_variant(p, j.base, j.cands[i], in, out, prod, pin, st, e0, e1, &ms,
-                          j.op_side);
-        if (rc) break;
+        const int r = time_variant(p, j.base, j.cands[i], in, out, prod, pin, st, e0, e1, &ms,
+                                   j.op_side);
+        if (r) return r;
         t[i] = ms < t[i] ? ms : t[i];
       }
-    if (rc) break;
     int best = j.cands[0];
     float best_ms = 1e30f;
     for (size_t i = 0; i < j.cands.size(); ++i)
       if (t[i] < best_ms) { best_ms = t[i]; best = j.cands[i]; }
+    if (tune_log) {
+      fprintf(stderr, "pfft autotune: pass %s%s\n", j.key.c_str(), prod ? " (after producer)" : "");
+      for (size_t i = 0; i < j.cands.size(); ++i)
+        fprintf(stderr, "  %9.4f ms %s%s\n", t[i], fast_name(j.cands[i]),
+                j.cands[i] == best ? "  <- chosen" : "");
+    }
     settle(*j.pp, j.op_side, best);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[j.key] = best;
+    return PFFT_OK;
+  };
+  for (Job& j : jobs) {
+    rc = tune(j, j.prod);
+    if (rc) break;
   }
   cleanup();
+  if (!rc) rc = refine_in_chain(p, jobs, tune_log);
   return rc;
 }
 
