@@ -68,6 +68,7 @@ struct PassArgs {
   int64_t lsout[kMaxDims];
   int64_t ntiles;      // ceil(ldim[0]/T) * prod(ldim[1..D-1])
   int32_t tile_order;  // 0: tile index runs along line dim 0 first, 1: along dims 1.. first
+  int32_t tile_rev;    // 1: tiles taken in descending order
   int64_t Mp;          // LDS line pitch (complex elements) of the work buffers
   int64_t raw_pitch;   // LDS line pitch (scalars) of the raw input tile
   int64_t lds_w1_off;  // byte offset of work buffer 1 (== raw tile) in LDS

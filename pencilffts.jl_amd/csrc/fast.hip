@@ -1233,6 +1233,7 @@ using namespace fk;
 // tile order (tile_split order 1) of a transposing kernel.
 constexpr int kOrderBit = 1 << 20;
 static int variant_index(int variant) { return (variant & (kOrderBit - 1)) - 1; }
+static_assert(kVariantReverse > kOrderBit, "variant id bits");
 
 static bool all_even(const int64_t* s, int from, int D) {
   for (int d = from; d < D; ++d)
@@ -1299,6 +1300,7 @@ void fast_apply(PassPlan& pp, int variant) {
   const Entry& e = registry()[variant_index(variant)];
   PassArgs& a = pp.args;
   a.tile_order = (variant & kOrderBit) ? 1 : 0;
+  a.tile_rev = (variant & kVariantReverse) ? 1 : 0;
   // contiguous passes: keep the input-adjacent line dim first (its partial
   // last tile costs little) unless it is shorter than a tile
   if (e.ld == IO_LINE && e.st == IO_LINE && a.ldim[0] < e.T) {
@@ -1369,7 +1371,7 @@ const char* fast_name(int variant) {
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
       (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
       (e.opk ? ",op" : "") + (e.pf ? ",PF" : "") +
-      ((variant & kOrderBit) ? ",ord1" : "") +
+      ((variant & kOrderBit) ? ",ord1" : "") + ((variant & kVariantReverse) ? ",rev" : "") +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();

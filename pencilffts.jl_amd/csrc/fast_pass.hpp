@@ -5,6 +5,9 @@
 
 namespace pfft {
 
+// Variant id bit: the pass walks its tiles in descending order.
+constexpr int kVariantReverse = 1 << 21;
+
 // Specialised kernels able to run a pass, as variant ids (first = default).
 // op: variants that carry a spectral operator (pfft_execute_op)
 std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op = false);

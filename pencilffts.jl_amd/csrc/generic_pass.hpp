@@ -29,8 +29,11 @@ struct TileCtx {
 // over line dims 1..D-1).  Order 0 walks line dim 0 first (consecutive tiles
 // are neighbouring blocks of one run); order 1 walks the other dims first, so
 // that workgroups running at the same time touch addresses spread along
-// those dims rather than one far-strided run.
+// those dims rather than one far-strided run.  tile_rev walks the tiles in
+// descending order: a pass then first reads what the pass before it wrote
+// last, while that is still in the device's last-level cache.
 PFFT_HD void tile_split(const PassArgs& a, int64_t tile, int64_t* b0, int64_t* rest) {
+  if (a.tile_rev) tile = a.ntiles - 1 - tile;
   const int64_t nb0 = (a.ldim[0] + a.T - 1) / a.T;
   if (a.tile_order == 0) {
     *b0 = tile % nb0;

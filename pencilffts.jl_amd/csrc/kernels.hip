@@ -150,6 +150,7 @@ struct TuneJob {
   std::string key;
   int op_side;
   const PassPlan* prod;   // the pass writing this one's input (nullptr: none)
+  std::vector<float> t1;  // phase-1 time of each candidate
 };
 
 // One pass of the chain on the given buffers (the timing loops of the tuner).
@@ -255,6 +256,19 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     if (j.op_side) continue;
     const int lst = j.pp >= p.fwd.data() && j.pp < p.fwd.data() + p.fwd.size() ? 0 : 1;
     const int idx = (int)(j.pp - (lst == 0 ? p.fwd.data() : p.bwd.data()));
+    // the phase-1 leaders, each walking its tiles both ways: the direction
+    // decides what the pass finds in the last-level cache, which only the
+    // chain shows
+    std::vector<size_t> order(j.cands.size());
+    for (size_t c = 0; c < order.size(); ++c) order[c] = c;
+    if (j.t1.size() == j.cands.size())
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t x, size_t y) { return j.t1[x] < j.t1[y]; });
+    std::vector<int> cands;
+    for (size_t c = 0; c < order.size() && c < 4; ++c) {
+      cands.push_back(j.cands[order[c]]);
+      cands.push_back(j.cands[order[c]] | kVariantReverse);
+    }
     const int vop = j.pp->variant_op;
     auto apply = [&](int var) {
       *j.pp = j.base;
@@ -262,10 +276,10 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
       j.pp->variant_op = vop;
       return fast_prepare(*j.pp);
     };
-    std::vector<float> t(j.cands.size(), 1e30f);
+    std::vector<float> t(cands.size(), 1e30f);
     for (int round = 0; round < rounds && !rc; ++round)
-      for (size_t c = 0; c < j.cands.size() && !rc; ++c) {
-        rc = apply(j.cands[c]);
+      for (size_t c = 0; c < cands.size() && !rc; ++c) {
+        rc = apply(cands[c]);
         if (!rc) rc = run_pairs(lst, idx, 1);   // warm-up
         if (!rc) rc = run_pairs(lst, idx, kPairs);
         if (rc) break;
@@ -281,18 +295,18 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
       }
     if (rc) break;
     size_t best = 0;
-    for (size_t c = 1; c < j.cands.size(); ++c)
+    for (size_t c = 1; c < cands.size(); ++c)
       if (t[c] < t[best]) best = c;
     if (log) {
       fprintf(stderr, "pfft autotune (in chain): %s pass %d\n", lst == 0 ? "forward" : "backward",
               idx + 1);
-      for (size_t c = 0; c < j.cands.size(); ++c)
-        fprintf(stderr, "  %9.4f ms %s%s\n", t[c], fast_name(j.cands[c]),
+      for (size_t c = 0; c < cands.size(); ++c)
+        fprintf(stderr, "  %9.4f ms %s%s\n", t[c], fast_name(cands[c]),
                 c == best ? "  <- chosen" : "");
     }
-    rc = apply(j.cands[best]);
+    rc = apply(cands[best]);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[j.key] = j.cands[best];
+    g_tune_cache[j.key] = cands[best];
   }
   cleanup();
   return rc;
@@ -310,8 +324,9 @@ static int autotune(Plan& p) {
   auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side, const PassPlan* prod) {
     std::vector<int> c = fast_candidates(p, base, op_side != 0);
     if (c.empty()) return;
+    const int rev = (p.flags & PFFT_FLAG_REVERSE_TILES) ? kVariantReverse : 0;
     if (forced) {
-      settle(pp, op_side, c[(forced - 1) % c.size()]);
+      settle(pp, op_side, c[(forced - 1) % c.size()] | rev);
       return;
     }
     const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "");
@@ -321,10 +336,10 @@ static int autotune(Plan& p) {
       return;
     }
     if (c.size() == 1 || (p.flags & PFFT_FLAG_NO_AUTOTUNE)) {
-      settle(pp, op_side, c[0]);
+      settle(pp, op_side, c[0] | rev);
       return;
     }
-    jobs.push_back({&pp, base, c, key, op_side, prod});
+    jobs.push_back({&pp, base, c, key, op_side, prod, {}});
     in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
     out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
     if (prod) {
@@ -418,6 +433,7 @@ static int autotune(Plan& p) {
     float best_ms = 1e30f;
     for (size_t i = 0; i < j.cands.size(); ++i)
       if (t[i] < best_ms) { best_ms = t[i]; best = j.cands[i]; }
+    j.t1 = t;
     if (tune_log) {
       fprintf(stderr, "pfft autotune: pass %s%s\n", j.key.c_str(), prod ? " (after producer)" : "");
       for (size_t i = 0; i < j.cands.size(); ++i)

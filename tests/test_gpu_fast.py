@@ -136,8 +136,9 @@ def test_prefetching_tile_loads(pf, gpu, dims, real_type, permute):
     n = int(np.prod(dims))
     tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
     for name, k in seen.items():
-        err_f, err_b, meta = run_case(pf, dims, (tr,), real_type, permute=permute, seed=k,
-                                      flags=L.FLAG_CANDIDATE(k))
-        print(name, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
-        assert name in meta.kernels
-        assert err_f <= tol and err_b <= tol, (name, err_f, err_b)
+        for rev in (0, L.FLAG_REVERSE_TILES):
+            err_f, err_b, meta = run_case(pf, dims, (tr,), real_type, permute=permute, seed=k,
+                                          flags=L.FLAG_CANDIDATE(k) | rev)
+            print(name, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+            assert any(x.startswith(name[:-1]) for x in meta.kernels)
+            assert err_f <= tol and err_b <= tol, (name, rev, err_f, err_b)

@@ -137,8 +137,8 @@ def test_gradient_of_a_mode(pf, gpu):
                          ids=["256F32", "96F32", "512F64", "1024F32"])
 def test_every_operator_variant(pf, gpu, dims, real_type):
     """Each operator-carrying variant the tuner may pick (forced with
-    PFFT_FLAG_CANDIDATE) on both sides: mul! with the derivative and ldiv!
-    with the inverse Laplacian, against the oracle."""
+    PFFT_FLAG_CANDIDATE, tiles walked both ways) on both sides: mul! with the
+    derivative and ldiv! with the inverse Laplacian, against the oracle."""
     T = pf.Transforms
     S = pf.spectral
     L = pf._lib
@@ -148,9 +148,10 @@ def test_every_operator_variant(pf, gpu, dims, real_type):
     tol = tol_for(CODE_NAMES[real_type], dims)
     m1 = OP.to_memory(multiplier(1, 2, dims, True), meta.perm_out)
     m3 = OP.to_memory(multiplier(3, 0, dims, True), meta.perm_out)
-    for k in range(6):
-        plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, real_type,
-                                flags=L.FLAG_CANDIDATE(k))
+    for k in range(24):
+        # candidates 0..11 in both tile directions
+        flags = L.FLAG_CANDIDATE(k % 12) | (L.FLAG_REVERSE_TILES if k >= 12 else 0)
+        plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, real_type, flags=flags)
         u = pf.allocate_input(plan)
         v = pf.allocate_output(plan)
         w = pf.allocate_input(plan)

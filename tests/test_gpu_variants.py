@@ -54,3 +54,25 @@ def test_every_candidate(pf, gpu, case):
     if flags:
         assert any("mfma" in x for x in seen), seen
     print(len(seen), "kernels:", sorted(seen))
+
+
+REVERSED = [CASES[1], CASES[2], CASES[3], CASES[8], CASES[10]]
+
+
+@pytest.mark.parametrize("case", REVERSED, ids=lambda c: "%s-%s-%s-f%d" % (
+    "x".join(map(str, c[0])), c[1], "F64" if c[2] else "F32", c[3]))
+def test_every_candidate_tiles_reversed(pf, gpu, case):
+    """The same candidates walking their tiles in descending order
+    (PFFT_FLAG_REVERSE_TILES), the direction the autotuner may pick per pass."""
+    dims, name, real_type, flags = case
+    T = pf.Transforms
+    L = pf._lib
+    n = int(np.prod(dims))
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
+    seen = set()
+    for k in range(MAX_CANDIDATES):
+        err_f, err_b, meta = run_case(pf, dims, _trs(T, name), real_type, seed=k,
+                                      flags=flags | L.FLAG_REVERSE_TILES | L.FLAG_CANDIDATE(k))
+        seen.update(meta.kernels)
+        assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
+    assert any(",rev" in x for x in seen), seen
