@@ -150,6 +150,7 @@ struct TuneJob {
   std::string key;
   int op_side;
   const PassPlan* prod;   // the pass writing this one's input (nullptr: none)
+  int list, index;        // position in the plan: list 0 forward, 1 backward
   std::vector<float> t1;  // phase-1 time of each candidate
 };
 
@@ -254,8 +255,7 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
   for (TuneJob& j : jobs) {
     if (rc) break;
     if (j.op_side) continue;
-    const int lst = j.pp >= p.fwd.data() && j.pp < p.fwd.data() + p.fwd.size() ? 0 : 1;
-    const int idx = (int)(j.pp - (lst == 0 ? p.fwd.data() : p.bwd.data()));
+    const int lst = j.list, idx = j.index;
     // the phase-1 leaders, each walking its tiles both ways: the direction
     // decides what the pass finds in the last-level cache, which only the
     // chain shows
@@ -321,7 +321,8 @@ static int autotune(Plan& p) {
     if (op_side) pp.variant_op = v;
     else fast_apply(pp, v);
   };
-  auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side, const PassPlan* prod) {
+  auto consider = [&](PassPlan& pp, const PassPlan& base, int op_side, const PassPlan* prod,
+                      int list, int index) {
     std::vector<int> c = fast_candidates(p, base, op_side != 0);
     if (c.empty()) return;
     const int rev = (p.flags & PFFT_FLAG_REVERSE_TILES) ? kVariantReverse : 0;
@@ -339,7 +340,7 @@ static int autotune(Plan& p) {
       settle(pp, op_side, c[0] | rev);
       return;
     }
-    jobs.push_back({&pp, base, c, key, op_side, prod, {}});
+    jobs.push_back({&pp, base, c, key, op_side, prod, list, index, {}});
     in_max = std::max(in_max, (size_t)pp.in_elems * eltype_bytes(pp.elt_in));
     out_max = std::max(out_max, (size_t)pp.out_elems * eltype_bytes(pp.elt_out));
     if (prod) {
@@ -349,9 +350,11 @@ static int autotune(Plan& p) {
   };
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (auto* list : {&p.fwd, &p.bwd})
-      for (size_t i = 0; i < list->size(); ++i)
-        consider((*list)[i], (*list)[i], 0, i > 0 ? &(*list)[i - 1] : nullptr);
+    for (int l = 0; l < 2; ++l) {
+      std::vector<PassPlan>& list = l == 0 ? p.fwd : p.bwd;
+      for (size_t i = 0; i < list.size(); ++i)
+        consider(list[i], list[i], 0, i > 0 ? &list[i - 1] : nullptr, l, (int)i);
+    }
     // operator carriers (when the plan admits spectral operators)
     const pfft_spectral_op probe = {PFFT_OP_DERIVATIVE, 0, 1.0, 0};
     for (int dir : {PFFT_FORWARD, PFFT_BACKWARD}) {
@@ -369,7 +372,8 @@ static int autotune(Plan& p) {
       q.lds_bytes = q.gen_lds;
       q.grid = q.gen_grid;
       q.variant = 0;
-      consider(pp, q, side, idx > 0 ? &list[idx - 1] : nullptr);
+      consider(pp, q, side, idx > 0 ? &list[idx - 1] : nullptr, dir == PFFT_FORWARD ? 0 : 1,
+               idx);
     }
   }
   if (jobs.empty()) return PFFT_OK;
