@@ -184,10 +184,11 @@ static int prepare_pass(const Plan& p, const PassPlan& pp) {
                                  : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);
 }
 
-// Second tuning phase: every plain pass's candidates again, each timed inside
-// whole mul! + ldiv! pairs on buffers of the plan's sizes (the other passes
-// at their current choice), so the caches, clocks and neighbouring kernels
-// are those of the pipeline.  Out-of-place plans; skipped when the buffers
+// Second tuning phase: every plain pass's leading candidates again, judged by
+// the time of whole mul! + ldiv! pairs on buffers of the plan's sizes (the
+// other passes at their current choice).  The caches, clocks and neighbouring
+// kernels are those of the pipeline, and a choice that speeds its own pass
+// but slows the next one loses.  Out-of-place plans; skipped when the buffers
 // do not fit.
 static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
   if (p.inplace) return PFFT_OK;
@@ -196,7 +197,7 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
   if (!any) return PFFT_OK;
   void *u = nullptr, *v = nullptr, *w = nullptr;
   hipStream_t st = nullptr;
-  const int kPairs = 3;
+  const int kPairs = 5;
   hipEvent_t e0[kPairs] = {}, e1[kPairs] = {};
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
@@ -220,10 +221,10 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
   }
   (void)hipMemsetD32Async((hipDeviceptr_t)u, 0x3f800000u, p.in_bytes / 4, st);
   void* bufs[3] = {nullptr, w, w ? (char*)w + p.work0_bytes : nullptr};
-  // runs `pairs` mul! + ldiv! pairs, recording events around pass `idx` of
-  // list `lst` (0 fwd, 1 bwd) in pair i
-  auto run_pairs = [&](int lst, int idx, int pairs) -> int {
+  // runs `pairs` mul! + ldiv! pairs, recording events around pair i
+  auto run_pairs = [&](int pairs) -> int {
     for (int i = 0; i < pairs; ++i) {
+      if (i < kPairs) HIP_TRY(hipEventRecord(e0[i], st));
       for (int d = 0; d < 2; ++d) {
         const std::vector<PassPlan>& passes = d == 0 ? p.fwd : p.bwd;
         void* in = d == 0 ? u : v;
@@ -233,13 +234,11 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
           const PassPlan& pp = passes[k];
           const void* src = pp.src == 0 ? in : bufs[pp.src];
           void* dst = pp.dst == 0 ? out : bufs[pp.dst];
-          const bool timed = d == lst && k == idx && i < kPairs;
-          if (timed) HIP_TRY(hipEventRecord(e0[i], st));
           const int r = launch_pass(p, pp, src, dst, d == 1 && k == np - 1, st);
           if (r) return r;
-          if (timed) HIP_TRY(hipEventRecord(e1[i], st));
         }
       }
+      if (i < kPairs) HIP_TRY(hipEventRecord(e1[i], st));
     }
     HIP_TRY(hipGetLastError());
     return PFFT_OK;
@@ -280,8 +279,8 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     for (int round = 0; round < rounds && !rc; ++round)
       for (size_t c = 0; c < cands.size() && !rc; ++c) {
         rc = apply(cands[c]);
-        if (!rc) rc = run_pairs(lst, idx, 1);   // warm-up
-        if (!rc) rc = run_pairs(lst, idx, kPairs);
+        if (!rc) rc = run_pairs(1);   // warm-up
+        if (!rc) rc = run_pairs(kPairs);
         if (rc) break;
         if (hipEventSynchronize(e1[kPairs - 1]) != hipSuccess) {
           rc = fail(PFFT_ERR_DEVICE, "autotune: event synchronisation failed");
@@ -294,12 +293,15 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
         t[c] = std::min(t[c], ts[kPairs / 2]);
       }
     if (rc) break;
+    // cands[0] is the first phase's choice: another candidate replaces it
+    // only when the pairs run clearly faster (pair times scatter by about 1 %)
     size_t best = 0;
     for (size_t c = 1; c < cands.size(); ++c)
       if (t[c] < t[best]) best = c;
+    if (t[best] > 0.985f * t[0]) best = 0;
     if (log) {
-      fprintf(stderr, "pfft autotune (in chain): %s pass %d\n", lst == 0 ? "forward" : "backward",
-              idx + 1);
+      fprintf(stderr, "pfft autotune (in chain, ms per pair): %s pass %d\n",
+              lst == 0 ? "forward" : "backward", idx + 1);
       for (size_t c = 0; c < cands.size(); ++c)
         fprintf(stderr, "  %9.4f ms %s%s\n", t[c], fast_name(cands[c]),
                 c == best ? "  <- chosen" : "");
@@ -454,7 +456,9 @@ static int autotune(Plan& p) {
     if (rc) break;
   }
   cleanup();
-  if (!rc) rc = refine_in_chain(p, jobs, tune_log);
+  // PFFT_TUNE_CHAIN=0 keeps the first phase's choices
+  const char* chain_env = getenv("PFFT_TUNE_CHAIN");
+  if (!rc && !(chain_env && chain_env[0] == '0')) rc = refine_in_chain(p, jobs, tune_log);
   return rc;
 }
 
