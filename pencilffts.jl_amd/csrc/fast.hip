@@ -89,6 +89,39 @@ __device__ __forceinline__ void dft16(C<R>* v) {
   }
 }
 
+// Good-Thomas prime-factor DFT of N1 N2 points, N1 and N2 coprime: with the
+// index maps n = (N2 n1 + N1 n2) mod N and k = (N2 <1/N2 mod N1> k1 + N1 <1/N1
+// mod N2> k2) mod N it is N1 DFTs of N2 points, then N2 DFTs of N1 points,
+// with no twiddles; the maps are compile-time register renamings.
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+template <typename R, int RAD>
+__device__ __forceinline__ void dft(C<R>* v);
+template <typename R, int N1, int N2>
+__device__ __forceinline__ void dft_pfa(C<R>* v) {
+  constexpr int N = N1 * N2;
+  constexpr int E1 = N2 * inv_mod(N2 % N1, N1), E2 = N1 * inv_mod(N1 % N2, N2);
+  C<R> t[N1][N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1)
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) t[n1][n2] = v[(N2 * n1 + N1 * n2) % N];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) dft<R, N2>(t[n1]);
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    C<R> u[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) u[n1] = t[n1][k2];
+    dft<R, N1>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) v[(E1 * k1 + E2 * k2) % N] = u[k1];
+  }
+}
+
 template <typename R, int RAD>
 __device__ __forceinline__ void dft(C<R>* v) {
   if constexpr (RAD == 2) dft2(v);
@@ -97,6 +130,11 @@ __device__ __forceinline__ void dft(C<R>* v) {
   else if constexpr (RAD == 5) dft5(v);
   else if constexpr (RAD == 8) dft8(v);
   else if constexpr (RAD == 16) dft16(v);
+  else if constexpr (RAD == 6) dft_pfa<R, 2, 3>(v);
+  else if constexpr (RAD == 10) dft_pfa<R, 2, 5>(v);
+  else if constexpr (RAD == 12) dft_pfa<R, 4, 3>(v);
+  else if constexpr (RAD == 20) dft_pfa<R, 4, 5>(v);
+  else static_assert(RAD == 0, "no register DFT of this radix");
 }
 
 // ---- one Stockham stage over one line ------------------------------------------
@@ -454,12 +492,14 @@ __device__ __forceinline__ int fft1024_mfma(C<float>* v, C<float>* line, int l,
     }                                                                    \
   };
 PFFT_CFG(32, 4, 8, 8, 4)
-PFFT_CFG(48, 4, 12, 4, 4, 3)
-PFFT_CFG(96, 8, 12, 4, 4, 3, 2)
-PFFT_CFG(160, 8, 20, 4, 4, 2, 5)
-PFFT_CFG(192, 16, 12, 4, 4, 4, 3)
-PFFT_CFG(320, 16, 20, 4, 4, 4, 5)
-PFFT_CFG(384, 32, 12, 4, 4, 4, 3, 2)
+// composite radices 6, 10, 12, 20 (prime-factor DFTs in registers) save
+// LDS round trips over schedules of 2, 3, 4 and 5
+PFFT_CFG(48, 4, 12, 4, 12)
+PFFT_CFG(96, 8, 12, 4, 4, 6)
+PFFT_CFG(160, 8, 20, 4, 4, 10)
+PFFT_CFG(192, 16, 12, 4, 4, 12)
+PFFT_CFG(320, 16, 20, 4, 4, 20)
+PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 #undef PFFT_CFG
 
 // kinds of the fast pass:
