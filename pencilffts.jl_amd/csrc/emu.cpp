@@ -18,6 +18,7 @@ using namespace pfft;
 
 struct pfft_emu_plan {
   Plan p;
+  int32_t tile_order = 0, tile_rev = 0;   // tile traversal (tile_split)
 };
 
 template <typename R>
@@ -103,10 +104,21 @@ __attribute__((visibility("default"))) int pfft_emu_execute_op(pfft_emu_plan* h,
       a.scale = 1.0 / p.scale;
     }
     a.adjoint = (dir == PFFT_ADJOINT_FORWARD || dir == PFFT_ADJOINT_BACKWARD) ? 1 : 0;
+    a.tile_order = h->tile_order;
+    a.tile_rev = h->tile_rev;
     if (i == op_pass) op_fill(p, dir, *op, op_side, a);
     if (p.real_type == PFFT_F32) emu_pass<float>(pp, a);
     else emu_pass<double>(pp, a);
   }
+  return PFFT_OK;
+}
+
+// Tile traversal of every pass (the specialised kernels' tuned tile order and
+// direction), to check on the host that each order covers every line once.
+__attribute__((visibility("default"))) int pfft_emu_set_traversal(pfft_emu_plan* h,
+                                                                 int32_t order, int32_t rev) {
+  h->tile_order = order ? 1 : 0;
+  h->tile_rev = rev ? 1 : 0;
   return PFFT_OK;
 }
 

@@ -28,12 +28,14 @@ class Emulator:
         L.pfft_emu_last_error.restype = C.c_char_p
         L.pfft_emu_nbytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.pfft_emu_execute.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.pfft_emu_set_traversal.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
         L.pfft_emu_execute_op.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
 
-    def run(self, dims, kinds, real_type, direction, x, permute=True, extra=()):
+    def run(self, dims, kinds, real_type, direction, x, permute=True, extra=(), traversal=(0, 0)):
         """x: numpy array holding the column-major memory image of the input
-        of `direction`; returns the output memory image (flat)."""
+        of `direction`; returns the output memory image (flat).  traversal:
+        (tile order, descending) of every pass."""
         N = len(dims)
         h = C.c_void_p()
         ne = len(extra)
@@ -43,6 +45,7 @@ class Emulator:
         if st:
             raise RuntimeError(self.lib.pfft_emu_last_error().decode())
         try:
+            self.lib.pfft_emu_set_traversal(h, *traversal)
             bi, bo = C.c_size_t(), C.c_size_t()
             self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
             nin, nout = (bi.value, bo.value) if direction in (0, 4) else (bo.value, bi.value)
@@ -95,7 +98,8 @@ class Emulator:
         finally:
             self.lib.pfft_emu_plan_destroy(h)
 
-    def run_op(self, dims, kinds, real_type, direction, x, op, out=None, permute=True, extra=()):
+    def run_op(self, dims, kinds, real_type, direction, x, op, out=None, permute=True, extra=(),
+               traversal=(0, 0)):
         """pfft_emu_execute_op: `op` is a ctypes pfft_spectral_op; `out` (uint8,
         optional) is the initial output for accumulate."""
         N = len(dims)
@@ -107,6 +111,7 @@ class Emulator:
         if st:
             raise RuntimeError(self.lib.pfft_emu_last_error().decode())
         try:
+            self.lib.pfft_emu_set_traversal(h, *traversal)
             bi, bo = C.c_size_t(), C.c_size_t()
             self.lib.pfft_emu_nbytes(h, C.byref(bi), C.byref(bo))
             nin, nout = (bi.value, bo.value) if direction in (0, 4) else (bo.value, bi.value)

@@ -90,3 +90,38 @@ def test_index_map_bit_exact(pf, emu):
         want = OP.memory_image(OP.to_memory(OP.from_image(vals, tuple(dims) + tuple(extra)),
                                             meta.perm_out))
         assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("dims,trs_name,permute,extra", [
+    ((16, 12, 6), "RFFT", True, ()), ((16, 12, 6), "FFT", True, (3,)),
+    ((9, 7, 5, 12), "RFFT", True, ()), ((11, 7, 13), "FFT", False, ()),
+    ((12, 10, 8), "RFFT", True, (3, 2))],
+    ids=["16x12x6-RFFT", "16x12x6-FFT-extra3", "9x7x5x12-RFFT", "11x7x13-FFT-noperm",
+         "12x10x8-RFFT-extra3x2"])
+def test_tile_traversals(pf, emu, dims, trs_name, permute, extra):
+    """Every tile order and direction the autotuner may give a pass
+    (tile_split: line dim 0 first or last, ascending or descending) covers
+    each line exactly once: outputs bit-identical to the default traversal,
+    forward, backward and with a fused spectral operator."""
+    T = pf.Transforms
+    meta, full = meta_for(T, dims, (getattr(T, trs_name)(),), "Float64", permute, extra)
+    kinds = [t.code for t in full]
+    rng = np.random.default_rng(5)
+    x = random_logical(rng, tuple(dims) + tuple(extra), meta.eltype_in)
+    xin = OP.memory_image(OP.to_memory(x, meta.perm_in)).astype(NP_DTYPES[meta.eltype_in])
+    op = pf._lib.SpectralOp(pf._lib.OP_LAPLACIAN, 0, 1.0, 0)
+    runs = {}
+    for trav in [(0, 0), (1, 0), (0, 1), (1, 1)]:
+        y = emu.run(dims, kinds, 1, 0, xin, permute, extra, traversal=trav)
+        b = emu.run(dims, kinds, 1, 2, y.view(NP_DTYPES[meta.eltype_out]), permute, extra,
+                    traversal=trav)
+        o = None
+        if len(dims) >= 2 and permute:
+            o = emu.run_op(dims, kinds, 1, 0, xin, op, permute=permute, extra=extra,
+                           traversal=trav)
+        runs[trav] = (y, b, o)
+    ref = runs[(0, 0)]
+    for trav, got in runs.items():
+        for a, r in zip(got, ref):
+            if r is not None:
+                assert np.array_equal(a, r), trav
