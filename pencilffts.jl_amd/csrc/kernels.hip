@@ -209,6 +209,13 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     }
     if (st) (void)hipStreamDestroy(st);
   };
+  // plan-sized scratch only while it leaves most of the device free
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+      2 * (p.in_bytes + p.out_bytes + p.work_bytes) > free_b) {
+    (void)hipGetLastError();
+    return PFFT_OK;
+  }
   bool ok = hipMalloc(&u, p.in_bytes) == hipSuccess && hipMalloc(&v, p.out_bytes) == hipSuccess &&
             (!p.work_bytes || hipMalloc(&w, p.work_bytes) == hipSuccess) &&
             hipStreamCreate(&st) == hipSuccess;
