@@ -1243,6 +1243,7 @@ static const std::vector<Entry>& registry() {
     add_dct<float, 512>(r);
     add_dct<double, 256>(r);
     add_dct<double, 64>(r);
+    add_size<double, 32>(r);   // r2c / c2r of 64 reals (the 64^3 Float64 case)
     add_size<double, 64>(r);
     add_size<double, 128>(r);
     add_size<double, 256>(r);

@@ -26,6 +26,7 @@ CASES = [
     ((256, 32, 16), "RFFT", 1, 0),
     ((512, 16, 16), "FFT", 1, 0),
     ((64, 32, 256), "MIXED", 0, 0),
+    ((64, 64, 32), "RFFT", 1, 0),
 ]
 
 
