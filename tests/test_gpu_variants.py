@@ -27,6 +27,8 @@ CASES = [
     ((512, 16, 16), "FFT", 1, 0),
     ((64, 32, 256), "MIXED", 0, 0),
     ((64, 64, 32), "RFFT", 1, 0),
+    ((32, 64, 256), "MIXED", 1, 0),
+    ((128, 32, 64), "MIXED", 1, 0),
 ]
 
 
