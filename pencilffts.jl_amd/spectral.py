@@ -48,3 +48,38 @@ def mul_adjoint_(dst, p, src, op, dim=0, alpha=1.0):
 def bmul_adjoint_(dst, p, src, op, dim=0, alpha=1.0, accumulate=False):
     """dst (+)= alpha conj(m) B^T(src): the transpose of bmul_ with the operator."""
     return p._execute(L.ADJOINT_BACKWARD, src, dst, op=spectral_op(op, dim, alpha, accumulate))
+
+
+# ---- vector calculus from the fused derivative (FourierOperations.jl) ------
+# `scale[d]` = 2 pi / L_d turns the integer wavenumbers of a 2 pi box into
+# those of a box of length L_d along logical dimension d.
+
+def divergence_(dst, p, us, scale=(1.0, 1.0, 1.0)):
+    """dst = F(div u) = sum_j i k_j F(u_j): the spectrum of the divergence of
+    the vector field us (test/include/FourierOperations.jl:18-42), by
+    accumulating forwards."""
+    for j, u in enumerate(us):
+        mul_(dst, p, u, DERIVATIVE, dim=j, alpha=scale[j], accumulate=j > 0)
+    return dst
+
+
+def curl_(dsts, p, us, scale=(1.0, 1.0, 1.0)):
+    """dsts[i] = F(curl u)_i: i k_j F(u_k) - i k_k F(u_j) for (i, j, k) cyclic
+    (test/include/FourierOperations.jl:44-86), two accumulating forwards per
+    component."""
+    for i in range(3):
+        j, k = (i + 1) % 3, (i + 2) % 3
+        mul_(dsts[i], p, us[k], DERIVATIVE, dim=j, alpha=scale[j])
+        mul_(dsts[i], p, us[j], DERIVATIVE, dim=k, alpha=-scale[k], accumulate=True)
+    return dsts
+
+
+def gradient_(dsts, p, uh, scale=None):
+    """dsts[j] = d_j u from the spectrum uh: one normalised backward transform
+    per component with the derivative on its first pass
+    (docs/examples/gradient.jl:254-273)."""
+    n = len(dsts)
+    scale = scale or (1.0,) * n
+    for j in range(n):
+        ldiv_(dsts[j], p, uh, DERIVATIVE, dim=j, alpha=scale[j])
+    return dsts
