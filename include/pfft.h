@@ -233,6 +233,18 @@ int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* 
                     const void* in, void* out, void* work, size_t work_bytes,
                     void* hip_stream);
 
+/* Divergence-free projection of the spectrum of a 3-component vector field,
+ * in place: u_i <- u_i - k_i (k . u) / |k|^2, with the mean mode (k = 0) left
+ * as is -- project_divergence_free! of the reference's Navier-Stokes example
+ * (docs/examples/navier_stokes.jl:351-366).  k are the plan's wavenumbers
+ * (those of the spectral operators) times scale[d] = 2 pi / L_d (scale NULL:
+ * a 2 pi box).  uh_x, uh_y, uh_z are output-shaped arrays (extra dims
+ * included) of a 3D plan of FFT/BFFT/RFFT dimensions.  It mixes the
+ * components at each wavenumber, so it is one elementwise kernel, not an
+ * operator carried by a transform pass.                                    */
+int pfft_project_divergence_free(pfft_plan* plan, void* uh_x, void* uh_y, void* uh_z,
+                                 const double* scale, void* hip_stream);
+
 /* Kernel timing of the last execute on a plan with profiling enabled: the
  * mirror of the reference's TimerOutputs sections "FFT" per stage
  * (operations.jl:250).  Enabled with pfft_plan_set_timing(plan, 1).        */

@@ -64,6 +64,7 @@ SIGNATURES = {
     "pfft_execute": (C.c_int, [vp, i32, vp, vp, vp, sz, vp]),
     "pfft_execute_op": (C.c_int, [vp, i32, C.POINTER(SpectralOp), vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
+    "pfft_project_divergence_free": (C.c_int, [vp, vp, vp, vp, P(C.c_double), vp]),
     "pfft_plan_set_timing": (C.c_int, [vp, i32]),
     "pfft_plan_kernels": (C.c_int, [vp, i32, C.c_char_p, sz]),
     "pfft_plan_stage_times": (C.c_int, [vp, i32, P(C.c_float)]),

@@ -269,6 +269,13 @@ int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* 
                                    (hipStream_t)hip_stream, op->op ? op : nullptr);)
 }
 
+int pfft_project_divergence_free(pfft_plan* plan, void* uh_x, void* uh_y, void* uh_z,
+                                 const double* scale, void* hip_stream) {
+  NEED(plan);
+  void* const uh[3] = {uh_x, uh_y, uh_z};
+  PFFT_GUARD(return device_project(plan->p, uh, scale, (hipStream_t)hip_stream);)
+}
+
 int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
                       const void* const* ins, void* const* outs, void* work, size_t work_bytes,
                       void* hip_stream) {

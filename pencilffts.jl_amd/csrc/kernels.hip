@@ -539,6 +539,91 @@ void device_release(Plan& p) {
   }
 }
 
+// ---- divergence-free projection (pfft_project_divergence_free) -----------------
+struct ProjArgs {
+  int64_t n[3];       // output memory dims of the transformed axes
+  int64_t rest;       // product of the extra dims
+  int32_t ld[3];      // logical dim of each memory axis
+  int32_t mode[3];    // per logical dim: 1 rfftfreq, 0 fftfreq
+  int64_t kn[3];      // per logical dim: spectrum length
+  double sc[3];       // per logical dim: 2 pi / L
+};
+
+template <typename R>
+__device__ __forceinline__ R proj_k(const ProjArgs& a, int d, int64_t c) {
+  const int64_t n = a.kn[d];
+  const int64_t k = a.mode[d] ? c : (c < (n + 1) / 2 ? c : c - n);
+  return (R)((double)k * a.sc[d]);
+}
+
+// One thread per (memory axis 0, 1) element; the other axes in a loop.
+// The projection reads all three components before writing any.
+template <typename R>
+__global__ __launch_bounds__(256) void k_project(cpx<R>* __restrict__ ux, cpx<R>* __restrict__ uy,
+                                                 cpx<R>* __restrict__ uz, ProjArgs a) {
+  const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m0 >= a.n[0]) return;
+  const int64_t n01 = a.n[0] * a.n[1];
+  const int64_t outer = a.n[2] * a.rest;
+  for (int64_t m1 = blockIdx.y; m1 < a.n[1]; m1 += gridDim.y) {
+    R k[3];
+    k[a.ld[0]] = proj_k<R>(a, a.ld[0], m0);
+    k[a.ld[1]] = proj_k<R>(a, a.ld[1], m1);
+    for (int64_t r = blockIdx.z; r < outer; r += gridDim.z) {
+      k[a.ld[2]] = proj_k<R>(a, a.ld[2], r % a.n[2]);
+      const R k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+      if (k2 == R(0)) continue;
+      const int64_t e = m0 + a.n[0] * m1 + n01 * r;
+      const cpx<R> x = ux[e], y = uy[e], z = uz[e];
+      // (k . u) / |k|^2, complex
+      const R inv = R(1) / k2;
+      const cpx<R> s = mk((k[0] * x.re + k[1] * y.re + k[2] * z.re) * inv,
+                          (k[0] * x.im + k[1] * y.im + k[2] * z.im) * inv);
+      ux[e] = mk(x.re - k[0] * s.re, x.im - k[0] * s.im);
+      uy[e] = mk(y.re - k[1] * s.re, y.im - k[1] * s.im);
+      uz[e] = mk(z.re - k[2] * s.re, z.im - k[2] * s.im);
+    }
+  }
+}
+
+int device_project(Plan& p, void* const* uh, const double* scale, hipStream_t stream) {
+  if (p.N != 3)
+    return fail(PFFT_ERR_ARGUMENT, "the divergence-free projection needs a 3D plan");
+  for (int d = 0; d < 3; ++d) {
+    const int32_t b = kind_base(p.kinds[d]);
+    if (b != PFFT_FFT && b != PFFT_BFFT && b != PFFT_RFFT)
+      return fail(PFFT_ERR_UNSUPPORTED,
+                  "the divergence-free projection needs wavenumbers: FFT/BFFT/RFFT dimensions");
+    if (!uh[d]) return fail(PFFT_ERR_ARGUMENT, "null component array");
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 3; ++j)
+      if (uh[i] == uh[j]) return fail(PFFT_ERR_ARGUMENT, "the three components must be distinct");
+  ProjArgs a;
+  const Stage& s = p.stages.back();
+  for (int i = 0; i < 3; ++i) {
+    a.ld[i] = s.perm[i];
+    a.n[i] = p.size_out[s.perm[i]];
+  }
+  a.rest = p.E;
+  for (int d = 0; d < 3; ++d) {
+    a.mode[d] = kind_base(p.kinds[d]) == PFFT_RFFT ? 1 : 0;
+    a.kn[d] = p.size_out[d];
+    a.sc[d] = scale ? scale[d] : 1.0;
+  }
+  const dim3 block(256);
+  const dim3 grid((unsigned)((a.n[0] + 255) / 256), (unsigned)std::min<int64_t>(a.n[1], 65535),
+                  (unsigned)std::min<int64_t>(a.n[2] * a.rest, 65535));
+  if (p.real_type == PFFT_F32)
+    hipLaunchKernelGGL(k_project<float>, grid, block, 0, stream, (cpx<float>*)uh[0],
+                       (cpx<float>*)uh[1], (cpx<float>*)uh[2], a);
+  else
+    hipLaunchKernelGGL(k_project<double>, grid, block, 0, stream, (cpx<double>*)uh[0],
+                       (cpx<double>*)uh[1], (cpx<double>*)uh[2], a);
+  HIP_TRY(hipGetLastError());
+  return PFFT_OK;
+}
+
 int device_set_timing(Plan& p, bool on) {
   int r = device_prepare(p);
   if (r) return r;

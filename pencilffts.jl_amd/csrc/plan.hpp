@@ -118,6 +118,7 @@ void device_release(Plan& p);
 int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
                    hipStream_t stream, const pfft_spectral_op* op = nullptr);
 int device_set_timing(Plan& p, bool on);
+int device_project(Plan& p, void* const* uh, const double* scale, hipStream_t stream);
 int device_stage_times(Plan& p, int n, float* ms);
 int device_pass_names(Plan& p, int dir, std::string* out);
 size_t eltype_bytes(int32_t eltype);

@@ -15,6 +15,8 @@ multiplier rides on the pass that writes the spectrum (mul!, the transpose of
 bmul!) or reads it (bmul!/ldiv!, the transpose of mul!), so it costs no memory
 pass.  `accumulate` adds into dst on the writing side.
 """
+import ctypes as C
+
 from . import _lib as L
 
 DERIVATIVE, LAPLACIAN, INV_LAPLACIAN, DEALIAS = (L.OP_DERIVATIVE, L.OP_LAPLACIAN,
@@ -83,3 +85,21 @@ def gradient_(dsts, p, uh, scale=None):
     for j in range(n):
         ldiv_(dsts[j], p, uh, DERIVATIVE, dim=j, alpha=scale[j])
     return dsts
+
+
+def project_divergence_free_(uhs, p, scale=None, stream=None):
+    """In place: uh_i <- uh_i - k_i (k . uh) / |k|^2 on the three output-shaped
+    spectra uhs (docs/examples/navier_stokes.jl:351-366); the mean mode is
+    left as is.  One elementwise kernel (pfft_project_divergence_free)."""
+    import torch
+    if len(uhs) != 3:
+        raise L.ArgumentError(L.ERR_ARGUMENT,
+                              "the divergence-free projection takes three components")
+    for a in uhs:
+        if a.pencil != p.pencil_output or a.eltype != p.eltype_output:
+            raise L.ArgumentError(L.ERR_ARGUMENT, "unexpected dimensions of input/output data")
+    sc = (C.c_double * 3)(*scale) if scale is not None else None
+    s = stream if stream is not None else torch.cuda.current_stream(uhs[0].parent.device)
+    L.check(L.lib.pfft_project_divergence_free(p._h, *(a.data_ptr() for a in uhs), sc,
+                                               s.cuda_stream))
+    return uhs

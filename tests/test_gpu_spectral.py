@@ -205,3 +205,46 @@ def test_taylor_green_fused_curl_divergence(pf, gpu):
     torch.cuda.synchronize()
     gx = u0 * k0 * np.cos(k0 * x) * np.cos(k0 * y) * np.cos(k0 * z)
     assert np.max(np.abs(g[0].logical().cpu().numpy() - gx)) <= 1e-12
+
+
+@pytest.mark.parametrize("dims,real_type,extra", [((16, 12, 10), 1, ()), ((32, 24, 20), 0, ()),
+                                                  ((12, 10, 8), 1, (2,))],
+                         ids=["16x12x10-F64", "32x24x20-F32", "12x10x8-F64-extra2"])
+def test_project_divergence_free(pf, gpu, dims, real_type, extra):
+    """project_divergence_free! (docs/examples/navier_stokes.jl:351-366) on
+    random spectra of real fields: the numpy formula on the same input, zero
+    divergence after, idempotence, and a divergence-free field unchanged."""
+    T = pf.Transforms
+    S = pf.spectral
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, real_type, extra_dims=extra)
+    box = (2 * np.pi, 4 * np.pi, 2 * np.pi)
+    scale = tuple(2 * np.pi / b for b in box)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    us = [pf.allocate_input(plan) for _ in range(3)]
+    for a in us:
+        a.parent.normal_(generator=g)
+    uh = [plan * a for a in us]
+    torch.cuda.synchronize()
+    before = [a.logical().cpu().numpy().astype(np.complex128) for a in uh]
+    S.project_divergence_free_(uh, plan, scale)
+    torch.cuda.synchronize()
+    got = [a.logical().cpu().numpy().astype(np.complex128) for a in uh]
+    ks = [np.arange(dims[0] // 2 + 1) * scale[0],
+          np.fft.fftfreq(dims[1], 1.0 / dims[1]) * scale[1],
+          np.fft.fftfreq(dims[2], 1.0 / dims[2]) * scale[2]]
+    K = [k.reshape(k.shape + (1,) * len(extra)) for k in np.meshgrid(*ks, indexing="ij")]
+    k2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    kdotu = sum(Ki * ui for Ki, ui in zip(K, before)) / np.where(k2 > 0, k2, 1.0)
+    want = [np.where(k2 > 0, ui - Ki * kdotu, ui) for Ki, ui in zip(K, before)]
+    tol = 1e-13 if real_type == 1 else 1e-6
+    norm = np.sqrt(sum(np.sum(np.abs(w) ** 2) for w in want))
+    assert np.sqrt(sum(np.sum(np.abs(a - b) ** 2) for a, b in zip(got, want))) <= tol * norm
+    div = sum(Ki * ui for Ki, ui in zip(K, got))
+    assert np.sqrt(np.sum(np.abs(div) ** 2)) <= tol * norm * max(np.max(np.abs(ks[1])), 1)
+    # idempotent
+    S.project_divergence_free_(uh, plan, scale)
+    torch.cuda.synchronize()
+    again = [a.logical().cpu().numpy().astype(np.complex128) for a in uh]
+    assert np.sqrt(sum(np.sum(np.abs(a - b) ** 2) for a, b in zip(again, got))) <= tol * norm
+    with pytest.raises(pf.ArgumentError):
+        S.project_divergence_free_(uh[:2], plan, scale)
