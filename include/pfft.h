@@ -81,9 +81,13 @@ enum {
   PFFT_FLAG_NO_AUTOTUNE = 1u << 1,  /* take the heuristic tile shapes */
   PFFT_FLAG_MFMA_SPLIT16 = 1u << 2, /* radix-16 stages on MFMA, split-FP16 */
   PFFT_FLAG_MFMA_BF16X3 = 1u << 3,  /* radix-16 stages on MFMA, BF16x3 */
-  PFFT_FLAG_REVERSE_TILES = 1u << 4 /* with NO_AUTOTUNE or CANDIDATE(k): every specialised
-                                       pass walks its tiles in descending order (the
-                                       autotuner otherwise chooses the direction per pass) */
+  PFFT_FLAG_REVERSE_TILES = 1u << 4, /* with NO_AUTOTUNE or CANDIDATE(k): every specialised
+                                        pass walks its tiles in descending order (the
+                                        autotuner otherwise chooses the direction per pass) */
+  PFFT_FLAG_STREAM_STORES = 1u << 5   /* likewise: every specialised pass stores its output
+                                        with the nontemporal hint (otherwise tuned per pass);
+                                        no counterpart in the reference, like the other
+                                        kernel-choice flags */
 };
 /* Instead of autotuning, run candidate k (0-based, taken modulo the number
  * of candidates) of every pass that has specialised kernels: lets a test

@@ -15,6 +15,7 @@ F32, F64, C32, C64, NOTHING = 0, 1, 2, 3, -1
 FORWARD, BACKWARD, BACKWARD_SCALED, ADJOINT_FORWARD, ADJOINT_BACKWARD = 0, 1, 2, 3, 4
 FLAG_GENERIC_ONLY, FLAG_NO_AUTOTUNE, FLAG_MFMA_SPLIT16, FLAG_MFMA_BF16X3 = 1, 2, 4, 8
 FLAG_REVERSE_TILES = 16
+FLAG_STREAM_STORES = 32
 
 
 def FLAG_CANDIDATE(k):

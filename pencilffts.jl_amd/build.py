@@ -26,7 +26,7 @@ DEVICE_FLAGS = [
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include")]
 
-PRODUCT_SOURCES = ["kernels.hip", "fast.hip", "plan.cpp", "capi.cpp"]
+PRODUCT_SOURCES = ["kernels.hip", "fast.hip", "fast_nt.hip", "plan.cpp", "capi.cpp"]
 EMU_SOURCES = ["emu.cpp", "plan.cpp"]
 
 

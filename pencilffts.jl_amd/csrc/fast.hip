@@ -18,6 +18,16 @@
 // Kinds: complex FFT (the backward sign as conj(FFT(conj x))), r2c and c2r of
 // an even real length n via the n/2-point complex FFT with the split/merge
 // twiddle w_n^k fused in.
+//
+// The file is compiled twice.  As itself it holds the kernels and the host
+// side (candidates, launch, names); as fast_nt.hip (PFFT_FAST_NT = 1) it
+// holds the same kernels with every output store carrying the nontemporal
+// hint and exports only that kernel table, entry for entry in the same
+// order.  The hint is a compile-time choice: a run-time branch around each
+// store changes register allocation and scheduling of every kernel.
+#ifndef PFFT_FAST_NT
+#define PFFT_FAST_NT 0
+#endif
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -279,12 +289,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__constant__ float kCos16[16] = {
+static __constant__ float kCos16[16] = {
     1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
     0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
     -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
     0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
-__constant__ float kSin16[16] = {
+static __constant__ float kSin16[16] = {
     0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
     1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
     0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
@@ -513,6 +523,29 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4 };
 enum { IO_LINE = 0, IO_TILE = 1 };
 
+// Output stores, plain or with the nontemporal (streaming) hint: the next
+// pass reads the output only after the whole pass, and whether keeping it in
+// the caches helps depends on the pass and its neighbours, so the autotuner
+// chooses per pass.
+template <bool NT, typename R>
+__device__ __forceinline__ void st_line(C<R>* p, C<R> v) {
+  if constexpr (NT) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2{v.re, v.im}, (v2*)p);
+  } else {
+    *p = v;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st_vec(float4* p, float4 v) {
+  if constexpr (NT) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(v4{v.x, v.y, v.z, v.w}, (v4*)p);
+  } else {
+    *p = v;
+  }
+}
+
 // tile element (row k, line t) sits at column t ^ tile_xor(k) of row k.
 // LDS has 64 banks of 4 bytes; a 64-bit access is served 32 lanes per cycle,
 // a 128-bit one 16 lanes per cycle.
@@ -562,7 +595,7 @@ __device__ __forceinline__ tile_v4 swap_pair(tile_v4 v, bool sw) {
 // with compile-time trip counts, so a thread issues all its global loads
 // before the first LDS write; the partial last round is a separate guarded
 // step that keeps nothing live across its branch.
-template <typename R, int T, int ROWS, int NTHR, int SW, bool ACC = false>
+template <typename R, int T, int ROWS, int NTHR, int SW, bool NT, bool ACC = false>
 __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, int64_t base,
                                            int tid) {
   C<R>* out = (C<R>*)a.out;
@@ -579,7 +612,7 @@ __device__ __forceinline__ void tile_store(const C<R>* tile, const PassArgs& a, 
         v = *(float4*)&vd;
       }
     }
-    *dst = v;
+    st_vec<NT>(dst, v);
   };
   constexpr int EPV = sizeof(R) == 4 ? 2 : 1;      // complex per 16-byte vector
   constexpr int VPR = T / EPV;                     // vectors per row
@@ -711,7 +744,7 @@ struct TileStage {
 // transforming the current one, with the twiddles and split/merge factors
 // read from LDS so that no other global load waits behind the prefetch.
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
-          int OCC, int OPK, int PF>
+          int OCC, int OPK, int PF, int NT>
 __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L))
 __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) {
   constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
@@ -992,9 +1025,9 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) out[l + L * p] = Y[p];
+            for (int p = 0; p < P; ++p) st_line<NT != 0>(&out[l + L * p], Y[p]);
             if constexpr (KIND == K_R2C)
-              if (l == 0) out[M] = Y[P];
+              if (l == 0) st_line<NT != 0>(&out[M], Y[P]);
           }
         }
       } else if constexpr (KIND == K_C2R) {
@@ -1002,7 +1035,8 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         C<R>* out = (C<R>*)((R*)a.out + c.out_base + (int64_t)t * a.lsout[0]);
         if (active) {
 #pragma unroll
-          for (int p = 0; p < P; ++p) out[l + L * p] = scl(conj(v[q][p]), sc);
+          for (int p = 0; p < P; ++p)
+            st_line<NT != 0>(&out[l + L * p], scl(conj(v[q][p]), sc));
         }
       } else {
 #pragma unroll
@@ -1022,7 +1056,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
               for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p] + out[l + L * p];
             } else {
 #pragma unroll
-              for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p];
+              for (int p = 0; p < P; ++p) st_line<NT != 0>(&out[l + L * p], v[q][p]);
             }
           }
         }
@@ -1049,11 +1083,11 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         const int64_t ob = c.out_base + (int64_t)(h * RCH) * a.sk_out;
         constexpr int LAST_ROWS = ROWS_OUT - (NSP - 1) * RCH;
         if (OPK && a.op_acc) {
-          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW, true>(tile, a, ob, tid);
-          else tile_store<R, T, RCH, NTHR, SW, true>(tile, a, ob, tid);
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW, NT, true>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR, SW, NT, true>(tile, a, ob, tid);
         } else {
-          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW>(tile, a, ob, tid);
-          else tile_store<R, T, RCH, NTHR, SW>(tile, a, ob, tid);
+          if (h == NSP - 1) tile_store<R, T, LAST_ROWS, NTHR, SW, NT>(tile, a, ob, tid);
+          else tile_store<R, T, RCH, NTHR, SW, NT>(tile, a, ob, tid);
         }
       }
     }
@@ -1072,8 +1106,8 @@ struct Entry {
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
           int OCC, int OPK, int PF>
 static void launch_one(dim3 g, dim3 b, size_t lds, hipStream_t s, const PassArgs& a) {
-  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>), g, b, lds, s,
-                     a);
+  hipLaunchKernelGGL((k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF, PFFT_FAST_NT>),
+                     g, b, lds, s, a);
 }
 
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF = 0, int NSP = 1,
@@ -1095,7 +1129,7 @@ static void add(std::vector<Entry>& v) {
   e.pf = PF;
   e.threads = T / LPW * Cfg<M, W>::L;
   e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, PF>();
-  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>;
+  e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF, PFFT_FAST_NT>;
   e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>;
   v.push_back(e);
 }
@@ -1270,11 +1304,49 @@ static const std::vector<Entry>& registry() {
 
 using namespace fk;
 
+#if PFFT_FAST_NT
+// the kernel table with nontemporal output stores, read by the other object
+const std::vector<Entry>& fast_registry_nt() { return registry(); }
+#else
+const std::vector<Entry>& fast_registry_nt();
+
 // Variant ids: registry index + 1, with kOrderBit set for the alternative
-// tile order (tile_split order 1) of a transposing kernel.
+// tile order (tile_split order 1) of a transposing kernel, kVariantReverse
+// for the descending tile direction and kVariantStream for the twin kernel
+// whose output stores carry the nontemporal hint.
 constexpr int kOrderBit = 1 << 20;
 static int variant_index(int variant) { return (variant & (kOrderBit - 1)) - 1; }
-static_assert(kVariantReverse > kOrderBit, "variant id bits");
+static_assert(kVariantReverse > kOrderBit && kVariantStream > kVariantReverse,
+              "variant id bits");
+
+// The twin table matches the plain one entry for entry (both objects build it
+// from the same list); checked once, as a mismatch would launch the wrong
+// kernel shape.
+static bool twins_match() {
+  static const bool ok = [] {
+    const std::vector<Entry>& a = registry();
+    const std::vector<Entry>& b = fast_registry_nt();
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i) {
+      const Entry& x = a[i];
+      const Entry& y = b[i];
+      if (x.real_bytes != y.real_bytes || x.M != y.M || x.kind != y.kind || x.ld != y.ld ||
+          x.st != y.st || x.T != y.T || x.lpw != y.lpw || x.threads != y.threads ||
+          x.mf != y.mf || x.nsp != y.nsp || x.w != y.w || x.occ != y.occ || x.opk != y.opk ||
+          x.pf != y.pf || x.lds != y.lds || y.fn == x.fn)
+        return false;
+    }
+    return true;
+  }();
+  return ok;
+}
+
+// the table entry that runs a variant (the caller has range-checked it)
+static const Entry& entry_of(int variant) {
+  const std::vector<Entry>& reg =
+      (variant & kVariantStream) && twins_match() ? fast_registry_nt() : registry();
+  return reg[variant_index(variant)];
+}
 
 static bool all_even(const int64_t* s, int from, int D) {
   for (int d = from; d < D; ++d)
@@ -1338,7 +1410,9 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
 }
 
 void fast_apply(PassPlan& pp, int variant) {
-  const Entry& e = registry()[variant_index(variant)];
+  // without a matching twin table the plain kernel runs (and the name says so)
+  if (!twins_match()) variant &= ~kVariantStream;
+  const Entry& e = entry_of(variant);
   PassArgs& a = pp.args;
   a.tile_order = (variant & kOrderBit) ? 1 : 0;
   a.tile_rev = (variant & kVariantReverse) ? 1 : 0;
@@ -1377,7 +1451,7 @@ void fast_apply(PassPlan& pp, int variant) {
 }
 
 int fast_prepare(const PassPlan& pp) {
-  const Entry& e = registry()[variant_index(pp.variant)];
+  const Entry& e = entry_of(pp.variant);
   if (e.lds > 64 * 1024) {
     hipError_t r = hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)e.lds);
@@ -1393,7 +1467,7 @@ int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream) {
   if (pp.variant < 1 || idx < 0 || idx >= (int)reg.size())
     return fail(PFFT_ERR_UNSUPPORTED, "no specialised kernel variant " +
                                           std::to_string(pp.variant));
-  const Entry& e = reg[idx];
+  const Entry& e = entry_of(pp.variant);
   e.launch(dim3((unsigned)pp.grid), dim3(e.threads), e.lds, stream, a);
   return PFFT_OK;
 }
@@ -1413,9 +1487,12 @@ const char* fast_name(int variant) {
       (e.occ > 0 ? ",O=" + std::to_string(e.occ) : std::string()) +
       (e.opk ? ",op" : "") + (e.pf ? ",PF" : "") +
       ((variant & kOrderBit) ? ",ord1" : "") + ((variant & kVariantReverse) ? ",rev" : "") +
+      ((variant & kVariantStream) ? ",nt" : "") +
       (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
       ">";
   return s.c_str();
 }
+
+#endif  // PFFT_FAST_NT
 
 }  // namespace pfft

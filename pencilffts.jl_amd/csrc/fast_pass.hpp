@@ -7,6 +7,8 @@ namespace pfft {
 
 // Variant id bit: the pass walks its tiles in descending order.
 constexpr int kVariantReverse = 1 << 21;
+// Variant id bit: the pass stores its output with the nontemporal hint.
+constexpr int kVariantStream = 1 << 22;
 
 // Specialised kernels able to run a pass, as variant ids (first = default).
 // op: variants that carry a spectral operator (pfft_execute_op)

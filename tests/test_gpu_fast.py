@@ -136,7 +136,7 @@ def test_prefetching_tile_loads(pf, gpu, dims, real_type, permute):
     n = int(np.prod(dims))
     tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
     for name, k in seen.items():
-        for rev in (0, L.FLAG_REVERSE_TILES):
+        for rev in (0, L.FLAG_REVERSE_TILES, L.FLAG_REVERSE_TILES | L.FLAG_STREAM_STORES):
             err_f, err_b, meta = run_case(pf, dims, (tr,), real_type, permute=permute, seed=k,
                                           flags=L.FLAG_CANDIDATE(k) | rev)
             print(name, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))

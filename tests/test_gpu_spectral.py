@@ -149,8 +149,10 @@ def test_every_operator_variant(pf, gpu, dims, real_type):
     m1 = OP.to_memory(multiplier(1, 2, dims, True), meta.perm_out)
     m3 = OP.to_memory(multiplier(3, 0, dims, True), meta.perm_out)
     for k in range(24):
-        # candidates 0..11 in both tile directions
-        flags = L.FLAG_CANDIDATE(k % 12) | (L.FLAG_REVERSE_TILES if k >= 12 else 0)
+        # candidates 0..11 in both tile directions, the reversed ones with
+        # streaming stores
+        flags = L.FLAG_CANDIDATE(k % 12) | (
+            L.FLAG_REVERSE_TILES | L.FLAG_STREAM_STORES if k >= 12 else 0)
         plan = pf.PencilFFTPlan(dims, T.RFFT(), (1,), None, real_type, flags=flags)
         u = pf.allocate_input(plan)
         v = pf.allocate_output(plan)

@@ -66,7 +66,9 @@ REVERSED = [CASES[1], CASES[2], CASES[3], CASES[8], CASES[10]]
     "x".join(map(str, c[0])), c[1], "F64" if c[2] else "F32", c[3]))
 def test_every_candidate_tiles_reversed(pf, gpu, case):
     """The same candidates walking their tiles in descending order
-    (PFFT_FLAG_REVERSE_TILES), the direction the autotuner may pick per pass."""
+    (PFFT_FLAG_REVERSE_TILES), the direction the autotuner may pick per pass;
+    every other candidate also stores with the nontemporal hint
+    (PFFT_FLAG_STREAM_STORES), the other tuned bit."""
     dims, name, real_type, flags = case
     T = pf.Transforms
     L = pf._lib
@@ -75,7 +77,9 @@ def test_every_candidate_tiles_reversed(pf, gpu, case):
     seen = set()
     for k in range(MAX_CANDIDATES):
         err_f, err_b, meta = run_case(pf, dims, _trs(T, name), real_type, seed=k,
-                                      flags=flags | L.FLAG_REVERSE_TILES | L.FLAG_CANDIDATE(k))
+                                      flags=flags | L.FLAG_REVERSE_TILES | L.FLAG_CANDIDATE(k)
+                                      | (L.FLAG_STREAM_STORES if k % 2 else 0))
         seen.update(meta.kernels)
         assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
     assert any(",rev" in x for x in seen), seen
+    assert any(",nt" in x for x in seen), seen
