@@ -84,10 +84,14 @@ enum {
   PFFT_FLAG_REVERSE_TILES = 1u << 4, /* with NO_AUTOTUNE or CANDIDATE(k): every specialised
                                         pass walks its tiles in descending order (the
                                         autotuner otherwise chooses the direction per pass) */
-  PFFT_FLAG_STREAM_STORES = 1u << 5   /* likewise: every specialised pass stores its output
+  PFFT_FLAG_STREAM_STORES = 1u << 5,  /* likewise: every specialised pass stores its output
                                         with the nontemporal hint (otherwise tuned per pass);
                                         no counterpart in the reference, like the other
                                         kernel-choice flags */
+  PFFT_FLAG_DENSE_WORK = 1u << 6      /* intermediate buffers keep dense rows (by default a
+                                        buffer loaded as tiles by the next pass, or of rows of
+                                        8 KiB and more, has its rows padded by 32 elements);
+                                        results are bit-identical either way */
 };
 /* Instead of autotuning, run candidate k (0-based, taken modulo the number
  * of candidates) of every pass that has specialised kernels: lets a test

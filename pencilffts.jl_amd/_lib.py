@@ -16,6 +16,7 @@ FORWARD, BACKWARD, BACKWARD_SCALED, ADJOINT_FORWARD, ADJOINT_BACKWARD = 0, 1, 2,
 FLAG_GENERIC_ONLY, FLAG_NO_AUTOTUNE, FLAG_MFMA_SPLIT16, FLAG_MFMA_BF16X3 = 1, 2, 4, 8
 FLAG_REVERSE_TILES = 16
 FLAG_STREAM_STORES = 32
+FLAG_DENSE_WORK = 64
 
 
 def FLAG_CANDIDATE(k):

@@ -32,16 +32,17 @@ class Emulator:
         L.pfft_emu_execute_op.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
 
-    def run(self, dims, kinds, real_type, direction, x, permute=True, extra=(), traversal=(0, 0)):
+    def run(self, dims, kinds, real_type, direction, x, permute=True, extra=(), traversal=(0, 0),
+            flags=0):
         """x: numpy array holding the column-major memory image of the input
         of `direction`; returns the output memory image (flat).  traversal:
-        (tile order, descending) of every pass."""
+        (tile order, descending) of every pass; flags: plan flags."""
         N = len(dims)
         h = C.c_void_p()
         ne = len(extra)
         st = self.lib.pfft_emu_plan_create(
             C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
-            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), 0)
+            int(permute), ne, (C.c_int64 * max(ne, 1))(*extra), flags)
         if st:
             raise RuntimeError(self.lib.pfft_emu_last_error().decode())
         try:
