@@ -675,15 +675,18 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
   // elements).  Measured on MI355X: rows of 8 KiB and more (1024-point
   // complex lines) run 4 % faster per 1024^3 pair padded, whatever the
   // access; a buffer the next pass loads as tiles loads 6 % faster padded
-  // (the 512^3 c2r pass); a buffer of shorter rows written by tile stores
-  // stores 6 % slower padded, so those keep dense rows.  In-place plans
-  // share one buffer between pencils and are never padded.
+  // (the 512^3 c2r pass, 4 KiB rows); a buffer of shorter rows written by
+  // tile stores stores 6 % slower padded, so those keep dense rows, as do
+  // rows under 2 KiB (small plans, where 32 elements would be a large
+  // share).  In-place plans share one buffer between pencils and are never
+  // padded.
   auto work_pad = [&](const int32_t* perm, const int64_t* dims, int32_t elt,
                       int consumer_dim) -> int64_t {
     if (p.inplace || (p.flags & PFFT_FLAG_DENSE_WORK)) return 0;
     const int64_t len = dims[perm[0]];
+    const int64_t row = len * (int64_t)eltype_bytes(elt);
     const bool tile_load = perm[0] != consumer_dim;
-    if (!tile_load && len * eltype_bytes(elt) < 8192) return 0;
+    if (row < (tile_load ? 2048 : 8192)) return 0;
     return len + 32;
   };
   {
