@@ -258,6 +258,9 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     }
   const double pair_bytes = 2.0 * ((double)p.in_bytes + (double)p.out_bytes);
   const int rounds = pair_bytes > 8e9 ? 1 : 2;
+  // a candidate replaces the incumbent only when the pairs run faster by
+  // kMargin, twice (see below)
+  const float kMargin = 0.992f;
   for (TuneJob& j : jobs) {
     if (rc) break;
     if (j.op_side) continue;
@@ -304,7 +307,6 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     // only when the pairs run faster by kMargin, twice (the median pair time
     // of one trial scatters by a few tenths of a percent; a streaming store
     // that speeds up the next pass gains about 1 % of the pair)
-    const float kMargin = 0.992f;
     size_t best = 0;
     for (size_t c = 1; c < cands.size(); ++c)
       if (t[c] < t[best]) best = c;
@@ -335,6 +337,75 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     rc = apply(cands[best]);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[j.key] = cands[best];
+  }
+  // The streaming bits of all tuned passes together: a streaming store speeds
+  // up the pass after it, so the bits interact along the chain, and the pass
+  // by pass search above settles in whichever local optimum its visiting
+  // order meets first.  With the kernels and tile directions fixed, every
+  // assignment of the bits is timed (2^P pairs runs, P <= 6).
+  std::vector<TuneJob*> sj;
+  for (TuneJob& j : jobs)
+    if (!j.op_side) sj.push_back(&j);
+  const int P = (int)sj.size();
+  if (!rc && pair_bytes <= 8e9 && P >= 2 && P <= 6) {
+    std::vector<int> basev(P);
+    int cur = 0;
+    for (int i = 0; i < P; ++i) {
+      basev[i] = sj[i]->pp->variant & ~kVariantStream;
+      if (sj[i]->pp->variant & kVariantStream) cur |= 1 << i;
+    }
+    auto apply_mask = [&](int m) -> int {
+      for (int i = 0; i < P; ++i) {
+        TuneJob& j = *sj[i];
+        const int vop = j.pp->variant_op;
+        *j.pp = j.base;
+        fast_apply(*j.pp, basev[i] | (((m >> i) & 1) ? kVariantStream : 0));
+        j.pp->variant_op = vop;
+        const int r = fast_prepare(*j.pp);
+        if (r) return r;
+      }
+      return PFFT_OK;
+    };
+    std::vector<float> tm((size_t)1 << P, 1e30f);
+    auto time_mask = [&](int m) -> int {
+      int r = apply_mask(m);
+      if (!r) r = run_pairs(1);
+      if (!r) r = run_pairs(kPairs);
+      if (r) return r;
+      if (hipEventSynchronize(e1[kPairs - 1]) != hipSuccess)
+        return fail(PFFT_ERR_DEVICE, "autotune: event synchronisation failed");
+      float ts[kPairs];
+      for (int i = 0; i < kPairs; ++i)
+        if (hipEventElapsedTime(&ts[i], e0[i], e1[i]) != hipSuccess) ts[i] = 1e30f;
+      std::sort(ts, ts + kPairs);
+      tm[m] = std::min(tm[m], ts[kPairs / 2]);
+      return PFFT_OK;
+    };
+    for (int m = 0; m < (1 << P) && !rc; ++m) rc = time_mask(m);
+    int best = cur;
+    for (int m = 0; m < (1 << P); ++m)
+      if (tm[m] < tm[best]) best = m;
+    if (!rc && best != cur && tm[best] < kMargin * tm[cur]) {
+      const float firstb = tm[best], first0 = tm[cur];
+      tm[best] = tm[cur] = 1e30f;
+      for (int round = 0; round < 4 && !rc; ++round) {
+        rc = time_mask(best);
+        if (!rc) rc = time_mask(cur);
+      }
+      if (log)
+        fprintf(stderr,
+                "pfft autotune (streaming bits): mask %d %.4f -> %.4f ms vs mask %d %.4f -> "
+                "%.4f ms\n",
+                best, firstb, tm[best], cur, first0, tm[cur]);
+      if (tm[best] > kMargin * tm[cur]) best = cur;
+    } else {
+      best = cur;
+    }
+    if (!rc) rc = apply_mask(best);
+    if (!rc) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      for (int i = 0; i < P; ++i) g_tune_cache[sj[i]->key] = sj[i]->pp->variant;
+    }
   }
   cleanup();
   return rc;
