@@ -430,7 +430,11 @@ static int autotune(Plan& p) {
       settle(pp, op_side, c[(forced - 1) % c.size()] | rev);
       return;
     }
-    const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "");
+    // the chain position is part of the key: the in-chain phases choose with
+    // the pass's neighbours in place, so two passes of one signature may differ
+    const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "") +
+                            "@" + std::to_string(list) + "." + std::to_string(index) + "/" +
+                            std::to_string(list == 0 ? p.fwd.size() : p.bwd.size());
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       settle(pp, op_side, it->second);
