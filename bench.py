@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dims", type=int, default=512)
-    ap.add_argument("--transform", default="RFFT", choices=["RFFT", "FFT"])
+    ap.add_argument("--transform", default="RFFT", choices=["RFFT", "FFT", "REDFT10"],
+                    help="RFFT (the headline), FFT (complex data) or REDFT10 (a Chebyshev "
+                         "plan of real data, R2R(REDFT10) along every dimension)")
     ap.add_argument("--real", default="Float32", choices=["Float32", "Float64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
@@ -53,8 +55,13 @@ def parse():
     ap.add_argument("--no-checks", action="store_true",
                     help="skip the Float64-plan error and the copy-ceiling measurement "
                          "(profiled runs: only the pair's own kernels)")
-    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--export-wisdom", metavar="PATH",
+                    help="write the tune data of this run to PATH (pfft_tune_export)")
+    ap.add_argument("--import-wisdom", metavar="PATH",
+                    help="plan from the tune data in PATH: nothing is timed at plan time "
+                         "(profiled runs then trace only the pairs' own launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with two 1 GiB device copies: the known-size reference that "
                          "scales HBM byte counters (rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE)")
@@ -90,24 +97,37 @@ def cpu_baseline(dims, transform, real, reps):
     single = real == "Float32"
     # warm-up on a small case to load pocketfft plans
     OP.forward(OM.PlanMeta((16, 16, 16), (transform,)), np.zeros((16, 16, 16)))
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        v = OP.forward(meta, xm, method="fast", workers=workers, single=single)
-        OP.backward(meta, v, scaled=True, method="fast", workers=workers, single=single)
-        times.append(time.perf_counter() - t0)
-    ms = 1e3 * min(times)
+
+    def median_ms(fn, n, warm=True):
+        if warm:
+            fn()                  # warm-up at the full size
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        return round(1e3 * ts[len(ts) // 2], 3)
+
+    def pipeline(w):
+        v = OP.forward(meta, xm, method="fast", workers=w, single=single)
+        OP.backward(meta, v, scaled=True, method="fast", workers=w, single=single)
+
     # the same pair as one fused n-dimensional scipy.fft call per direction
     # (no pencil chain): a second CPU reference point
     import scipy.fft as sfft
-    tfused = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
+
+    def fused(w):
         if transform == "RFFT":
-            sfft.irfftn(sfft.rfftn(x, workers=workers), s=dims, workers=workers)
+            sfft.irfftn(sfft.rfftn(x, workers=w), s=dims, workers=w)
         else:
-            sfft.ifftn(sfft.fftn(x, workers=workers), workers=workers)
-        tfused.append(time.perf_counter() - t0)
+            sfft.ifftn(sfft.fftn(x, workers=w), workers=w)
+
+    ms = median_ms(lambda: pipeline(workers), reps)
+    ms_fused = median_ms(lambda: fused(workers), reps)
+    # one thread (the pocketfft plans are warm by now)
+    ms1 = median_ms(lambda: pipeline(1), reps, warm=False)
+    ms1_fused = median_ms(lambda: fused(1), reps, warm=False)
     # the reference's CPU-runnable case, forward and backward timed apart
     m64 = OM.PlanMeta((64, 64, 64), ("RFFT",), "Float64")
     x64 = np.asfortranarray(np.random.default_rng(42).uniform(0.0, 1.0, size=(64, 64, 64)))
@@ -128,15 +148,26 @@ def cpu_baseline(dims, transform, real, reps):
                     break
     except OSError:
         pass
-    return {"value": round(ms, 3), "unit": "ms/pair", "cores": workers, "kind": "port",
-            "sample": "%d full %s^3 %s %s pairs (oracle pipeline: permuted copy + scipy.fft per "
-                      "stage, %s, workers=%d), best of %d; host CPU: %s"
-                      % (reps, dims[0], real, transform, "float32" if single else "float64",
+    return {"value": ms, "unit": "ms/pair", "cores": workers, "kind": "port",
+            "sample": "full %s^3 %s %s pairs (oracle pipeline: permuted copy + scipy.fft "
+                      "(pocketfft; FFTW is absent from the image) per stage, %s, "
+                      "workers=%d), median of %d after a warm-up pair; host CPU: %s"
+                      % (dims[0], real, transform, "float32" if single else "float64",
                          workers, reps, cpu),
-            "fused_nd_ms_per_pair": round(1e3 * min(tfused), 3),
-            "small_64_f64": {"ms_forward": round(1e3 * min(tf), 4),
-                             "ms_backward": round(1e3 * min(tb), 4), "cores": 1,
-                             "sample": "64^3 Float64 RFFT / ldiv!, best of 20"}}
+            "cpu_model": cpu, "library": "scipy %s pocketfft" % _scipy_version(),
+            "fused_nd_ms_per_pair": ms_fused,
+            "workers_1": {"ms_per_pair": ms1, "fused_nd_ms_per_pair": ms1_fused, "cores": 1,
+                          "sample": "the same pipeline and fused pair on one thread, "
+                                    "median of %d" % reps},
+            "small_64_f64": {"ms_forward": round(1e3 * sorted(tf)[len(tf) // 2], 4),
+                             "ms_backward": round(1e3 * sorted(tb)[len(tb) // 2], 4),
+                             "cores": 1,
+                             "sample": "64^3 Float64 RFFT / ldiv!, median of 20"}}
+
+
+def _scipy_version():
+    import scipy
+    return scipy.__version__
 
 
 # ---------------------------------------------------------------- helpers
@@ -205,25 +236,59 @@ def pair_stats(torch, pf, plan, u, v, n):
             "max": round(max(t), 4)}
 
 
-def copy_ceiling(torch, nbytes=1 << 30, reps=10):
-    """Device-to-device copy rate (read + write bytes per second) of a 1 GiB
-    buffer: the measured ceiling beside the datasheet peak."""
+def bandwidth_lib():
+    """The streaming-copy kernel (pencilffts.jl_amd/csrc/bandwidth.hip)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "pencilffts.jl_amd", "lib", "libpfft_bandwidth.so")
+    lib = C.CDLL(path)
+    lib.pfft_bandwidth_copy.restype = C.c_int
+    lib.pfft_bandwidth_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                        C.c_void_p]
+    return lib
+
+
+def stream_copy(torch, lib, dst, src, per_cu):
+    st = torch.cuda.current_stream()
+    r = lib.pfft_bandwidth_copy(dst.data_ptr(), src.data_ptr(), src.numel() * 4, per_cu,
+                                st.cuda_stream)
+    if r:
+        raise RuntimeError("pfft_bandwidth_copy failed: hipError %d" % r)
+
+
+def copy_ceiling(torch, nbytes=2 << 30, reps=10):
+    """Measured HBM ceiling: read + write bytes per second of the
+    hand-written streaming copy (16-byte loads, nontemporal stores,
+    persistent grid) over a 2 GiB buffer, best over resident workgroups per
+    CU; torch's device copy of the same buffer beside it."""
+    lib = bandwidth_lib()
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return 2 * nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+
+    sweep = {}
+    for per_cu in (1, 2, 4, 8):
+        sweep[per_cu] = round(timed(lambda: stream_copy(torch, lib, dst, src, per_cu)), 1)
+    ok = bool(torch.equal(dst, src))
+    torch_gbs = timed(lambda: dst.copy_(src))
     del src, dst
     torch.cuda.empty_cache()
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+    best = max(sweep, key=sweep.get)
+    return {"GBs": sweep[best], "kernel": "k_stream_copy (16-byte loads, nontemporal stores, "
+                                          "%d workgroups of 256 per CU)" % best,
+            "bytes": 2 * nbytes, "per_cu_sweep_GBs": sweep, "copy_correct": ok,
+            "torch_copy_GBs": round(torch_gbs, 1)}
 
 
 def error_vs_f64(torch, pf, dims, tr, u32):
@@ -464,12 +529,18 @@ def main():
     T = pf.Transforms
     dims = (args.dims,) * 3
     real_type = T.Float32 if args.real == "Float32" else T.Float64
-    tr = T.RFFT() if args.transform == "RFFT" else T.FFT()
+    tr = {"RFFT": T.RFFT(), "FFT": T.FFT(), "REDFT10": T.R2R(T.REDFT10)}[args.transform]
+    if args.import_wisdom:
+        pf.import_wisdom(args.import_wisdom)
     plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, real_type, flags=args.flags)
     u = pf.allocate_input(plan)
     v = pf.allocate_output(plan)
     g = torch.Generator(device="cuda").manual_seed(42)
     fill(torch, u, g)
+    # tuning happens here (or nothing is timed, with imported tune data)
+    t0 = time.perf_counter()
+    plan.prepare()
+    tune_s = time.perf_counter() - t0
 
     t0 = time.perf_counter()
     ms = time_pairs(torch, pf, plan, u, v, args.steps, args.warmup)
@@ -510,7 +581,8 @@ def main():
         "dtype": "f32" if real_type == T.Float32 else "f64", "data": "synthetic",
         "config": {"workload": "%d^3 %s %s+%s pair (mul! + ldiv!)"
                                % (X, args.real, args.transform,
-                                  "BRFFT" if args.transform == "RFFT" else "BFFT"),
+                                  {"RFFT": "BRFFT", "FFT": "BFFT",
+                                   "REDFT10": "REDFT01"}[args.transform]),
                    "dims": list(dims), "transforms": args.transform, "real_type": args.real,
                    "input": "uniform[-1,1) seed 42" if args.transform == "RFFT"
                             else "re/im iid N(0,1) seed 42",
@@ -520,8 +592,20 @@ def main():
         "pair_ms_stats": stats,
         "pairs_per_s": round(1e3 / ms, 2),
         "wall_s": round(wall, 3),
+        "plan_prepare_s": round(tune_s, 3),
         "roofline": roofline,
     }
+    if not args.no_checks and not args.flags:
+        # the same pair on the first candidate of every pass (no timing at
+        # plan time: the choice of a plan made with FFTW.ESTIMATE and no
+        # tune data)
+        p0 = pf.PencilFFTPlan(dims, tr, (1, 1), None, real_type,
+                              flags=pf._lib.FLAG_NO_AUTOTUNE)
+        ms0 = time_pairs(torch, pf, p0, u, v, max(5, args.steps // 2), max(2, args.warmup // 2))
+        out["untuned"] = {"ms_per_pair": round(ms0, 4),
+                          "hbm_frac": round(pair_bytes / (ms0 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "kernels": p0.kernels(0) + p0.kernels(2)}
+        del p0
     # compulsory bytes of a fused 3D transform (read the input, write the
     # output once per direction): the lower bound beside the 3-pass figure
     comp = 2 * (u.parent.numel() * u.parent.element_size() +
@@ -535,19 +619,27 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_checks:
         cc = copy_ceiling(torch)
-        out["copy_ceiling_GBs"] = round(cc, 1)
-        out["pair_frac_of_copy_ceiling"] = round(pair_bytes / (ms * 1e-3) / 1e9 / cc, 4)
+        out["copy_ceiling"] = cc
+        out["copy_ceiling_GBs"] = cc["GBs"]
+        out["pair_frac_of_copy_ceiling"] = round(pair_bytes / (ms * 1e-3) / 1e9 / cc["GBs"], 4)
+        out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc["GBs"], 4)
     if args.copy_calibration:
+        # two 1 GiB streaming copies: the known-size launches that scale the
+        # HBM byte counters of a rocprofv3 --pmc run
+        lib = bandwidth_lib()
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(2):
-            dst.copy_(src)
+            stream_copy(torch, lib, dst, src, 4)
         torch.cuda.synchronize()
         del src, dst
     if not args.no_extras:
         out["extras"] = run_extras(torch, pf, max(5, args.steps // 5), max(2, args.warmup // 5),
                                    args.extra)
-    if not args.no_cpu:
+    if args.export_wisdom:
+        # the choices of every plan this run tuned (headline and extras)
+        pf.export_wisdom(args.export_wisdom)
+    if not args.no_cpu and args.transform in ("RFFT", "FFT"):
         out["cpu_baseline"] = cpu_baseline(dims, args.transform, args.real, args.cpu_reps)
     else:
         out["cpu_baseline"] = None

@@ -88,10 +88,18 @@ enum {
                                         with the nontemporal hint (otherwise tuned per pass);
                                         no counterpart in the reference, like the other
                                         kernel-choice flags */
-  PFFT_FLAG_DENSE_WORK = 1u << 6      /* intermediate buffers keep dense rows (by default a
+  PFFT_FLAG_DENSE_WORK = 1u << 6,     /* intermediate buffers keep dense rows (by default a
                                         buffer loaded as tiles by the next pass, or of rows of
                                         8 KiB and more, has its rows padded by 32 elements);
                                         results are bit-identical either way */
+  PFFT_FLAG_ESTIMATE = 1u << 7        /* plan without timing anything: each pass takes the
+                                        choice already in the tune cache (from tuning earlier
+                                        in the process, or pfft_tune_import), else its first
+                                        candidate.  The counterpart of planning with
+                                        fftw_flags = FFTW.ESTIMATE (plans.jl:241-250), the
+                                        reference's default; without it a plan is tuned on
+                                        its first execute, as FFTW.MEASURE / PATIENT plans
+                                        are timed at creation */
 };
 /* Instead of autotuning, run candidate k (0-based, taken modulo the number
  * of candidates) of every pass that has specialised kernels: lets a test
@@ -252,6 +260,23 @@ int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* 
  * operator carried by a transform pass.                                    */
 int pfft_project_divergence_free(pfft_plan* plan, void* uh_x, void* uh_y, void* uh_z,
                                  const double* scale, void* hip_stream);
+
+/* ---- tune data ("wisdom") -------------------------------------------------
+ * The autotuner's choices of this process as text, for a later process to
+ * plan the same shapes without timing anything: the counterpart of FFTW's
+ * wisdom (FFTW.export_wisdom / import_wisdom / forget_wisdom of FFTW.jl),
+ * which the reference's FFTW plans (plans.jl:523-553, built with fftw_flags,
+ * plans.jl:241-250) draw on.  pfft_tune_export writes the text into buf
+ * (NUL-terminated) when buf is non-NULL and large enough, and its size with
+ * the NUL into *needed when needed is non-NULL.  pfft_tune_import merges
+ * text exported by the same build on the same kind of device, and refuses
+ * any other with PFFT_ERR_ARGUMENT.  Imported choices apply to plans
+ * created afterwards, whatever their flags (PFFT_FLAG_NO_AUTOTUNE and
+ * PFFT_FLAG_CANDIDATE(k) aside), so a plan whose every pass is covered
+ * times nothing.  pfft_tune_forget empties the cache.                      */
+int pfft_tune_export(char* buf, size_t buflen, size_t* needed);
+int pfft_tune_import(const char* text);
+int pfft_tune_forget(void);
 
 /* Kernel timing of the last execute on a plan with profiling enabled: the
  * mirror of the reference's TimerOutputs sections "FFT" per stage

@@ -17,6 +17,11 @@ FLAG_GENERIC_ONLY, FLAG_NO_AUTOTUNE, FLAG_MFMA_SPLIT16, FLAG_MFMA_BF16X3 = 1, 2,
 FLAG_REVERSE_TILES = 16
 FLAG_STREAM_STORES = 32
 FLAG_DENSE_WORK = 64
+FLAG_ESTIMATE = 128
+
+# FFTW planner flags (FFTW.jl's constants), accepted by PencilFFTPlan
+FFTW_MEASURE, FFTW_EXHAUSTIVE, FFTW_PATIENT, FFTW_ESTIMATE, FFTW_WISDOM_ONLY = (
+    0, 1 << 3, 1 << 5, 1 << 6, 1 << 21)
 
 
 def FLAG_CANDIDATE(k):
@@ -67,6 +72,9 @@ SIGNATURES = {
     "pfft_execute_op": (C.c_int, [vp, i32, C.POINTER(SpectralOp), vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
     "pfft_project_divergence_free": (C.c_int, [vp, vp, vp, vp, P(C.c_double), vp]),
+    "pfft_tune_export": (C.c_int, [C.c_char_p, sz, P(sz)]),
+    "pfft_tune_import": (C.c_int, [C.c_char_p]),
+    "pfft_tune_forget": (C.c_int, []),
     "pfft_plan_set_timing": (C.c_int, [vp, i32]),
     "pfft_plan_kernels": (C.c_int, [vp, i32, C.c_char_p, sz]),
     "pfft_plan_stage_times": (C.c_int, [vp, i32, P(C.c_float)]),

@@ -3,6 +3,7 @@
 Produces
   lib/libpencilffts_hip.so  -- the product: kernels + plan + C ABI (include/pfft.h)
   lib/libpencilffts_emu.so  -- host emulator of the generic pass (tests only)
+  lib/libpfft_bandwidth.so  -- streaming-copy ceiling (benchmark only)
 
 Usage: python pencilffts.jl_amd/build.py [-v]
 """
@@ -60,9 +61,15 @@ def build(verbose=False):
         jobs.append([HIPCC] + COMMON + ["-x", "hip", "--offload-host-only", "-O2",
                                         "-c", os.path.join(CSRC, src), "-o", obj])
         eobjs.append(obj)
+    # streaming-copy ceiling of the benchmark (not part of the library)
+    bw_out = os.path.join(LIB, "libpfft_bandwidth.so")
+    jobs.append([HIPCC] + COMMON + DEVICE_FLAGS + ["-x", "hip", "-shared",
+                                                   os.path.join(CSRC, "bandwidth.hip"),
+                                                   "-o", bw_out + ".tmp"])
     with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
         for f in [ex.submit(_run, cmd, verbose) for cmd in jobs]:
             f.result()
+    os.replace(bw_out + ".tmp", bw_out)
     out = os.path.join(LIB, "libpencilffts_hip.so")
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-o", out + ".tmp"] + objs, verbose)
     os.replace(out + ".tmp", out)

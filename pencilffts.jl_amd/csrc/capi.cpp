@@ -306,6 +306,31 @@ int pfft_plan_kernels(pfft_plan* plan, int32_t direction, char* buf, size_t bufl
   })
 }
 
+int pfft_tune_export(char* buf, size_t buflen, size_t* needed) {
+  PFFT_GUARD({
+    std::string s;
+    int r = tune_export(&s);
+    if (r) return r;
+    if (needed) *needed = s.size() + 1;
+    if (!buf) return PFFT_OK;
+    if (buflen < s.size() + 1) return fail(PFFT_ERR_ARGUMENT, "buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return PFFT_OK;
+  })
+}
+
+int pfft_tune_import(const char* text) {
+  NEED(text);
+  PFFT_GUARD(return tune_import(text);)
+}
+
+int pfft_tune_forget(void) {
+  PFFT_GUARD({
+    tune_forget();
+    return PFFT_OK;
+  })
+}
+
 int pfft_plan_set_timing(pfft_plan* plan, int32_t enable) {
   NEED(plan);
   PFFT_GUARD(return device_set_timing(plan->p, enable != 0);)

@@ -30,6 +30,7 @@
 #endif
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <string>
 
 #include "../../include/pfft.h"
@@ -492,6 +493,149 @@ __device__ __forceinline__ int fft1024_mfma(C<float>* v, C<float>* line, int l,
   return e;
 }
 
+// ---- 1024 = 32 x 32 on MFMA, split-FP16, the line in registers -----------------------
+// Four-step form of a 1024-point line x[n1 + 32 n2] on one wavefront:
+//   Y[n1, k2] = sum_n2 x[n1 + 32 n2] w32^(n2 k2)       (32-point DFTs, MFMA)
+//   Y'        = Y[n1, k2] w1024^(n1 k2)                (twiddle, VALU)
+//   X[k2 + 32 k1] = sum_n1 Y'[n1, k2] w32^(n1 k1)      (32-point DFTs, MFMA)
+// Each step is one complex 32x32x32 product, as four real products on
+// v_mfma_f32_32x32x16_f16 over two K-chunks.  The first step takes the
+// data as the A operand (row n1 = lane % 32, K = n2 in registers), so its
+// accumulator holds Y with k2 across the lanes and n1 in registers -- the
+// layout the second step takes as its B operand (K = n1 in registers, column
+// k2 = lane % 32).  No LDS exchange and no transposition are needed inside
+// the line.  K slot (h = lane / 32, t, chunk c) stands for index
+// pi(8h + t + 16c) = 4h + (t & 3) + 8 (t >> 2) + 16c in both steps, which
+// is the accumulator row order; so the DFT-matrix slices the two steps need
+// are the same registers, and register p of lane l holds point
+//   J(l, p) = (l & 31) + 128 (l >> 5) + 32 (p & 3) + 256 (p >> 2)
+// both on input and on output.  Operands are split x = hi + lo in FP16
+// (round to nearest both times; hi_A hi_B + hi_A lo_B + lo_A hi_B in FP32).
+// The line is scaled on entry by 2^e with |x| 2^e < 2^9, so the first
+// step's outputs (sums of 32 complex terms) stay below 2^15 and in FP16
+// range for the second step.
+__device__ __forceinline__ constexpr int jmap_lane(int l) { return (l & 31) + 128 * (l >> 5); }
+__device__ __forceinline__ constexpr int jmap_off(int p) { return 32 * (p & 3) + 256 * (p >> 2); }
+
+static __constant__ float kCos32[32] = {
+    1.0f, 0.98078528040323044913f, 0.92387953251128675613f, 0.83146961230254523708f,
+    0.70710678118654752440f, 0.55557023301960222474f, 0.38268343236508977173f,
+    0.19509032201612826785f, 0.0f, -0.19509032201612826785f, -0.38268343236508977173f,
+    -0.55557023301960222474f, -0.70710678118654752440f, -0.83146961230254523708f,
+    -0.92387953251128675613f, -0.98078528040323044913f, -1.0f, -0.98078528040323044913f,
+    -0.92387953251128675613f, -0.83146961230254523708f, -0.70710678118654752440f,
+    -0.55557023301960222474f, -0.38268343236508977173f, -0.19509032201612826785f, 0.0f,
+    0.19509032201612826785f, 0.38268343236508977173f, 0.55557023301960222474f,
+    0.70710678118654752440f, 0.83146961230254523708f, 0.92387953251128675613f,
+    0.98078528040323044913f};
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// hi + lo split of 8 floats into two FP16 operand slices
+__device__ __forceinline__ void split8(const float (&x)[8], f16x8& hi, f16x8& lo) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const _Float16 h = (_Float16)x[t];
+    hi[t] = h;
+    lo[t] = (_Float16)(x[t] - (float)h);
+  }
+}
+__device__ __forceinline__ f16x8 neg8(f16x8 v) {
+  return __builtin_bit_cast(f16x8, __builtin_bit_cast(s16x8, v) ^ (short)0x8000);
+}
+
+struct Mfma32Const {   // this lane's slice of the 32-point DFT matrix, both steps
+  f16x8 rh[2], rl[2], ih[2], il[2];
+  __device__ void init(int l) {
+    const int j = l & 31, h = l >> 5;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int n = 4 * h + (t & 3) + 8 * (t >> 2) + 16 * c;
+        const int m = (j * n) & 31;
+        const float fr = kCos32[m];
+        const float fi = -kCos32[(m + 24) & 31];   // -sin(2 pi m / 32)
+        const _Float16 a = (_Float16)fr, b = (_Float16)fi;
+        rh[c][t] = a;
+        rl[c][t] = (_Float16)(fr - (float)a);
+        ih[c][t] = b;
+        il[c][t] = (_Float16)(fi - (float)b);
+      }
+  }
+};
+
+// D += A B with A, B split in hi + lo
+__device__ __forceinline__ f32x16 mm3(f16x8 ah, f16x8 al, f16x8 bh, f16x8 bl, f32x16 d) {
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, d, 0, 0, 0);
+  return d;
+}
+
+// v: point J(l, p) in register p on entry, X[J(l, p)] on exit (times 2^e).
+// tw2: the step twiddles, entry 64 i + l = w1024^((l & 31) n1), n1 the row
+// of accumulator register i.  Returns e.
+__device__ __forceinline__ int fft1024_mfma32(C<float>* v, int l, const C<float>* __restrict__ tw2,
+                                              const Mfma32Const& F) {
+  // scale: largest magnitude as bits (|x| orders like its bit pattern)
+  unsigned mb = 0;
+#pragma unroll
+  for (int p = 0; p < 16; ++p)
+    mb = max(mb, max(__float_as_uint(v[p].re) & 0x7fffffffu,
+                     __float_as_uint(v[p].im) & 0x7fffffffu));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o, 64));
+  const int e = mb ? 8 - ((int)(mb >> 23) - 127) : 0;
+  f32x16 dr = {}, di = {};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float xr[8], xi[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      xr[t] = ldexpf(v[8 * c + t].re, e);
+      xi[t] = ldexpf(v[8 * c + t].im, e);
+    }
+    f16x8 rh, rl, ih, il;
+    split8(xr, rh, rl);
+    split8(xi, ih, il);
+    // Re Y = Xr Fr - Xi Fi, Im Y = Xr Fi + Xi Fr (data as the A operand)
+    dr = mm3(rh, rl, F.rh[c], F.rl[c], dr);
+    dr = mm3(neg8(ih), neg8(il), F.ih[c], F.il[c], dr);
+    di = mm3(rh, rl, F.ih[c], F.il[c], di);
+    di = mm3(ih, il, F.rh[c], F.rl[c], di);
+  }
+  // twiddle w1024^(n1 k2)
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const C<float> w = tw2[64 * i + l];
+    const float a = dr[i], b = di[i];
+    dr[i] = a * w.re - b * w.im;
+    di[i] = a * w.im + b * w.re;
+  }
+  f32x16 zr = {}, zi = {};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float yr[8], yi[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      yr[t] = dr[8 * c + t];
+      yi[t] = di[8 * c + t];
+    }
+    f16x8 rh, rl, ih, il;
+    split8(yr, rh, rl);
+    split8(yi, ih, il);
+    // the DFT matrix as the A operand now
+    zr = mm3(F.rh[c], F.rl[c], rh, rl, zr);
+    zr = mm3(F.ih[c], F.il[c], neg8(ih), neg8(il), zr);
+    zi = mm3(F.rh[c], F.rl[c], ih, il, zi);
+    zi = mm3(F.ih[c], F.il[c], rh, rl, zi);
+  }
+#pragma unroll
+  for (int p = 0; p < 16; ++p) v[p] = mk(zr[p], zi[p]);
+  return e;
+}
+
 // mixed radix: P is a multiple of every radix of the schedule
 #define PFFT_CFG(MM, LL, PP, ...)                                        \
   template <> struct Cfg<MM> {                                           \
@@ -520,7 +664,24 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 //           split of the real/imaginary parts and the post-twiddle
 //   K_DCT3C REDFT01 of complex data: pre-twiddle over (k, M-k), backward
 //           M-point FFT, inverse Makhoul reorder
-enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4 };
+//   K_DST2C RODFT10 of complex data: y_k = DCT-II((-1)^j x_j)_{M-1-k}, the
+//           sign folded into the Makhoul gather, the reversal into the
+//           post-processing's reads
+//   K_DST3C RODFT01 of complex data: y_k = (-1)^k DCT-III(x_{M-1-j})_k, the
+//           reversal folded into the pre-twiddle's gather, the sign into the
+//           output factor
+//   K_DHTC  DHT of complex data: one M-point FFT of x = a + i b, then
+//           A = (Z_k + conj Z_{M-k}) / 2, B = (Z_k - conj Z_{M-k}) / 2i and
+//           y_k = (Re A - Im A) + i (Re B - Im B)
+// K_REAL_PAIRS (OR-ed into the R2R kinds): the same transforms of real data,
+// two real lines (2j, 2j + 1) forming the real and imaginary parts of one
+// complex line.  On the side where the pass is transposed the two lines are
+// adjacent in memory, so the tile moves them as complex elements; on a line
+// side the pair's real and imaginary parts are read from (written to) two
+// lines, a.lsin[0] / 2 (a.lsout[0] / 2) elements apart.
+enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4, K_DST2C = 5, K_DST3C = 6,
+       K_DHTC = 7 };
+constexpr int K_REAL_PAIRS = 16;
 enum { IO_LINE = 0, IO_TILE = 1 };
 
 // Output stores, plain or with the nontemporal (streaming) hint: the next
@@ -535,6 +696,11 @@ __device__ __forceinline__ void st_line(C<R>* p, C<R> v) {
   } else {
     *p = v;
   }
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void st_real(R* p, R v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 template <bool NT>
 __device__ __forceinline__ void st_vec(float4* p, float4 v) {
@@ -677,10 +843,12 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 // A transposed store goes out in NSP row chunks of M / NSP rows, so the
 // tile needs 1 / NSP of the rows in LDS and more workgroups fit on a CU.
 // PF: the prefetching variants keep the twiddles (and the c2r merge factors)
-// in LDS after the tile / line slots.
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int PF = 0>
+// in LDS after the tile / line slots.  The 32 x 32 MFMA lines (MF = 3) keep
+// the whole line in registers and use no line slot.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int PF = 0,
+          int MF = 0>
 constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per line
-  constexpr size_t lines = (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
+  constexpr size_t lines = MF == 3 ? 0 : (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
   constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
   constexpr int rows_st = M / NSP + (KIND == K_R2C ? 1 : 0);
   constexpr size_t tld = LD == IO_TILE ? (size_t)rows_ld * T * sizeof(C<R>) : 0;
@@ -743,21 +911,26 @@ struct TileStage {
 // next tile (into registers: the transposed tile, or its lines) before
 // transforming the current one, with the twiddles and split/merge factors
 // read from LDS so that no other global load waits behind the prefetch.
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int MF, int NSP, int W,
+template <typename R, int M, int T, int LPW, int KINDF, int LD, int ST, int MF, int NSP, int W,
           int OCC, int OPK, int PF, int NT>
 __global__ __launch_bounds__((T / LPW * Cfg<M, W>::L))
 __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) {
+  constexpr int KIND = KINDF & 15;
+  constexpr bool RP = (KINDF & K_REAL_PAIRS) != 0;   // real data, two lines per complex line
   constexpr int L = Cfg<M, W>::L, P = Cfg<M, W>::P;
   constexpr int TQ = T / LPW;
   constexpr int NTHR = TQ * L;
   static_assert(TQ * LPW == T, "LPW must divide T");
   static_assert(P % NSP == 0 && (NSP == 1 || ST == IO_TILE), "store chunks split P");
   static_assert(OPK == 0 || KIND == K_C2C, "spectral operators ride on c2c passes");
+  static_assert(!RP || KIND >= K_DCT2C, "real pairs: R2R kinds");
   // inputs / outputs per line (the r2c output and c2r input carry M + 1)
   constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
   constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
-  constexpr bool GATHER = KIND == K_C2R || KIND == K_DCT2C || KIND == K_DCT3C;  // pre reads (k, M-k)
-  constexpr bool POST = KIND == K_R2C || KIND == K_DCT2C || KIND == K_DCT3C;    // post reads LDS
+  constexpr bool DCTLIKE = KIND == K_DCT2C || KIND == K_DCT3C || KIND == K_DST2C ||
+                           KIND == K_DST3C;
+  constexpr bool GATHER = KIND == K_C2R || DCTLIKE;                 // pre reads (k, M-k)
+  constexpr bool POST = KIND == K_R2C || DCTLIKE || KIND == K_DHTC;  // post reads LDS
   constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
   extern __shared__ __align__(16) char smem[];
   C<R>* lds = (C<R>*)smem;
@@ -772,7 +945,16 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   // conflict-free tile swizzle when the line's lanes cover its period
   constexpr int SW = L % (sizeof(R) == 4 ? 32 : 16) == 0 && T * sizeof(C<R>) <= 256 ? 1 : 0;
   constexpr int SWP = SW ? (sizeof(R) == 4 ? 32 : 16) : (sizeof(R) == 4 ? T / 2 : T);
-  constexpr bool ADD8 = L % 8 == 0, ADDT = L % SWP == 0;
+  // MF = 3 (1024 = 32 x 32 on MFMA): register p of lane l holds point
+  // J(l, p) = jmap_lane(l) + jmap_off(p) instead of l + L p; the offsets are
+  // multiples of 32 rows, so the tile swizzle (period SWP | 32) keeps the
+  // lane-base addressing
+  constexpr bool JM = MF == 3;
+  static_assert(!JM || (KIND == K_C2C && OPK == 0 && PF == 0 && L == 64 && NSP <= 4),
+                "32 x 32 MFMA lines: c2c, one wavefront per line");
+  const int lrow = JM ? jmap_lane(l) : l;
+  auto ROW = [&](int p) -> int { return JM ? jmap_off(p) : L * p; };
+  constexpr bool ADD8 = L % 8 == 0, ADDT = (JM ? 32 : L) % SWP == 0;
   C<R>* const lp = line + pad8(l);
   C<R>* const lm = line + pad8(L - l);
   auto LK = [&](int p) -> C<R>& {   // line[pad8(l + L p)]
@@ -784,8 +966,8 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     else return line[pad8(M - l - L * p)];
   };
   auto TK = [&](int p, int t, int k0) -> C<R>& {   // tile[tile_idx(k0 + l + L p, t)]
-    if constexpr (ADDT) return tile[tile_idx<R, T, SW>(l, t) + (k0 + L * p) * T];
-    else return tile[tile_idx<R, T, SW>(k0 + l + L * p, t)];
+    if constexpr (ADDT) return tile[tile_idx<R, T, SW>(lrow, t) + (k0 + ROW(p)) * T];
+    else return tile[tile_idx<R, T, SW>(k0 + lrow + ROW(p), t)];
   };
   auto TMK = [&](int p, int t) -> C<R>& {          // tile[tile_idx(M - l - L p, t)]
     if constexpr (ADDT) return tile[tile_idx<R, T, SW>(L - l, t) + (M - L * (p + 1)) * T];
@@ -796,7 +978,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
                                            : KIND == K_C2C || KIND == K_R2C)),
                 "prefetching variants: c2c, c2r with transposed loads, r2c with line loads");
   constexpr bool PFT = PF && LD == IO_TILE, PFL = PF && LD == IO_LINE;
-  constexpr size_t TBL = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP>();
+  constexpr size_t TBL = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, 0, MF>();
   if constexpr (PF) {
     C<R>* st = (C<R>*)(smem + TBL);
     for (int i = tid; i < M; i += NTHR) st[i] = ((const C<R>*)a.tw)[i];
@@ -807,12 +989,15 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   const C<R>* __restrict__ tw = PF ? (const C<R>*)(smem + TBL) : (const C<R>*)a.tw;
   const C<R>* __restrict__ aux = PF ? (const C<R>*)(smem + TBL) + M : (const C<R>*)a.aux;
   // backward FFTs run as conj(FFT(conj x))
-  const bool conj_in = KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C);
+  const bool conj_in =
+      KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
   const bool conj_out = KIND == K_C2C && a.bwd != 0;
   const R sc = a.do_scale ? (R)a.scale : R(1);
   static_assert(MF == 0 || (M == 1024 && sizeof(R) == 4), "MFMA stages: 1024-point f32 lines");
-  MfmaA<MF == 0 ? 1 : MF> mfA;
-  if constexpr (MF != 0) mfA.init(l % 64);
+  MfmaA<MF == 1 || MF == 2 ? MF : 1> mfA;
+  if constexpr (MF == 1 || MF == 2) mfA.init(l % 64);
+  [[maybe_unused]] Mfma32Const mf32;
+  if constexpr (JM) mf32.init(l);
 
   [[maybe_unused]] TileStage<R, T, PFT ? ROWS_IN : 1, NTHR, SW> stg;
   // the next tile's lines (contiguous loads), as native vectors
@@ -873,9 +1058,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         // element x_k of this line, from the tile or from the line's LDS slot
         if constexpr (LD == IO_LINE) {
           if (q > 0) line_sync<L>();
-          const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
-          if (active)
-            for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = in[k];
+          if constexpr (RP) {
+            // the pair's two real lines, lsin[0] / 2 apart
+            const R* ia = (const R*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+            const R* ib = ia + a.lsin[0] / 2;
+            if (active)
+              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = mk(ia[k], ib[k]);
+          } else {
+            const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+            if (active)
+              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = in[k];
+          }
           line_sync<L>();
         }
         auto X = [&](int k) -> C<R> {
@@ -906,6 +1099,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           } else if constexpr (KIND == K_DCT2C) {
             // Makhoul reorder: v_j = x_{2j} (j < ceil(M/2)), v_{M-1-j} = x_{2j+1}
             z = X(2 * k < M ? 2 * k : 2 * M - 2 * k - 1);
+          } else if constexpr (KIND == K_DST2C) {
+            // the reorder of (-1)^j x_j: the odd-indexed points change sign
+            z = 2 * k < M ? X(2 * k) : scl(X(2 * M - 2 * k - 1), R(-1));
+          } else if constexpr (KIND == K_DST3C) {
+            // the pre-twiddle of the reversed line x'_j = x_{M-1-j}
+            const C<R> xk = X(M - 1 - k);
+            const C<R> xr = k == 0 ? mk(R(0), R(0)) : X(k - 1);
+            z = conj(aux[k]) * (xk + mul_mi(xr));
           } else {
             // V_k = w_4M^-k (x_k - i x_{M-k}), x_M = 0
             const C<R> xk = XK(p);
@@ -936,11 +1137,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
                              ? (const C<R>*)((const R*)a.in + c.in_base + (int64_t)t * a.lsin[0])
                              : (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
         if (active) {
+          // real pairs: the pair's two real lines, lsin[0] / 2 apart
+          [[maybe_unused]] const R* ia = (const R*)a.in + c.in_base + (int64_t)t * a.lsin[0];
 #pragma unroll
           for (int p = 0; p < P; ++p) {
             C<R> x;
             if constexpr (PFL) x = mk(nl[q][p].x, nl[q][p].y);
-            else x = in[l + L * p];
+            else if constexpr (RP) x = mk(ia[l + L * p], ia[a.lsin[0] / 2 + l + L * p]);
+            else x = in[lrow + ROW(p)];
             v[q][p] = conj_in ? conj(x) : x;
           }
           if constexpr (OPK) {
@@ -974,8 +1178,11 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       if (q > 0) line_sync<L>();
       [[maybe_unused]] R scq = sc;   // output scaling of this line
       if constexpr (MF != 0) {
-        const int e = fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l,
-                                          (const C<float>*)tw, mfA);
+        int e;
+        if constexpr (JM)
+          e = fft1024_mfma32((C<float>*)v[q], l, (const C<float>*)aux, mf32);
+        else
+          e = fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw, mfA);
         // remove the entry scaling 2^e: folded into the output factor when
         // that stays a normal number (sc >= 2^-60 here), else applied apart
         if (e >= -60 && e <= 60) {
@@ -991,13 +1198,34 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         // the transform lands in natural order in the line's LDS slot
         line_sync<L>();
 #pragma unroll
-        for (int p = 0; p < P; ++p) LK(p) = KIND == K_DCT3C ? conj(v[q][p]) : v[q][p];
+        for (int p = 0; p < P; ++p)
+          LK(p) = KIND == K_DCT3C || KIND == K_DST3C ? conj(v[q][p]) : v[q][p];
         line_sync<L>();
         C<R> Y[NX];
 #pragma unroll
         for (int p = 0; p < NX; ++p) {
           const int k = p < P ? l + L * p : M;
-          if constexpr (KIND == K_R2C) {
+          if constexpr (KIND == K_DST2C) {
+            // DCT-II post-processing at the reversed index kk = M - 1 - k
+            const int kk = M - 1 - k;
+            const C<R> Vk = line[pad8(kk)];
+            const C<R> Vn = conj(kk != 0 ? line[pad8(k + 1)] : line[0]);
+            const C<R> Va = scl(Vk + Vn, R(0.5));
+            const C<R> Vb = scl(mul_mi(Vk - Vn), R(0.5));
+            const C<R> w = aux[kk];
+            Y[p] = scl(mk(w.re * Va.re - w.im * Va.im, w.re * Vb.re - w.im * Vb.im), R(2) * sc);
+          } else if constexpr (KIND == K_DST3C) {
+            // inverse Makhoul reorder, then the sign (-1)^k
+            Y[p] = scl(line[pad8((k & 1) ? M - 1 - (k >> 1) : (k >> 1))], (k & 1) ? -sc : sc);
+          } else if constexpr (KIND == K_DHTC) {
+            // the transforms A, B of the real and imaginary parts, then
+            // Re - Im of each
+            const C<R> Z = LK(p);
+            const C<R> Zc = conj(k != 0 ? LMK(p) : line[0]);
+            const C<R> A = scl(Z + Zc, R(0.5));
+            const C<R> B = scl(mul_mi(Z - Zc), R(0.5));
+            Y[p] = scl(mk(A.re - A.im, B.re - B.im), sc);
+          } else if constexpr (KIND == K_R2C) {
             // X_k = (Z_k + conj Z_{M-k})/2 - i w_n^k (Z_k - conj Z_{M-k})/2, k = 0..M
             const C<R> Z = p < P ? LK(p) : line[0];
             const C<R> Zc = conj(p < P && k != 0 ? LMK(p) : line[0]);
@@ -1021,7 +1249,18 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         }
 #pragma unroll
         for (int p = 0; p < NX; ++p) v[q][p] = Y[p];
-        if constexpr (ST == IO_LINE) {
+        if constexpr (ST == IO_LINE && RP) {
+          // the pair's two real lines, lsout[0] / 2 apart
+          R* oa = (R*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+          R* ob = oa + a.lsout[0] / 2;
+          if (active) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              st_real<NT != 0>(&oa[l + L * p], Y[p].re);
+              st_real<NT != 0>(&ob[l + L * p], Y[p].im);
+            }
+          }
+        } else if constexpr (ST == IO_LINE) {
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
 #pragma unroll
@@ -1056,7 +1295,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
               for (int p = 0; p < P; ++p) out[l + L * p] = v[q][p] + out[l + L * p];
             } else {
 #pragma unroll
-              for (int p = 0; p < P; ++p) st_line<NT != 0>(&out[l + L * p], v[q][p]);
+              for (int p = 0; p < P; ++p) st_line<NT != 0>(&out[lrow + ROW(p)], v[q][p]);
             }
           }
         }
@@ -1128,7 +1367,7 @@ static void add(std::vector<Entry>& v) {
   e.opk = OPK;
   e.pf = PF;
   e.threads = T / LPW * Cfg<M, W>::L;
-  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, PF>();
+  e.lds = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, PF, MF>();
   e.fn = (const void*)k_fast<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF, PFFT_FAST_NT>;
   e.launch = launch_one<R, M, T, LPW, KIND, LD, ST, MF, NSP, W, OCC, OPK, PF>;
   v.push_back(e);
@@ -1152,6 +1391,20 @@ static void add_mfma(std::vector<Entry>& v) {
   add<float, 1024, 8, 2, K_C2C, IO_TILE, IO_LINE, MF, 1, 2>(v);
 }
 
+// c2c 1024-point passes with 1024 = 32 x 32 on MFMA in split-FP16, the line
+// in registers (one wavefront per line, no line slots in LDS)
+static void add_mfma32(std::vector<Entry>& v) {
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
+  add<float, 1024, 2, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
+  add<float, 1024, 16, 1, K_C2C, IO_LINE, IO_TILE, 3, 2>(v);
+  add<float, 1024, 16, 1, K_C2C, IO_LINE, IO_TILE, 3, 4>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 3>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 3, 2>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_TILE, IO_LINE, 3>(v);
+  add<float, 1024, 16, 1, K_C2C, IO_TILE, IO_LINE, 3>(v);
+}
+
 // c2c passes that carry a spectral operator (pfft_execute_op): one tile shape
 // per load/store form
 template <typename R, int M>
@@ -1167,18 +1420,30 @@ static void add_op(std::vector<Entry>& v) {
   add<R, M, TT, 1, K_C2C, IO_TILE, IO_LINE, 0, 1, 1, 0, 1>(v);
 }
 
-template <typename R, int M>
-static void add_dct(std::vector<Entry>& v) {
+// R2R kinds (complex data, and real data as line pairs): contiguous lines,
+// transposed stores, transposed loads
+template <typename R, int M, int K, bool MORE>
+static void add_r2r_kind(std::vector<Entry>& v) {
   constexpr int TL = 256 / Cfg<M>::L;
   constexpr int TT = sizeof(R) == 4 ? 16 : 8;
-  add<R, M, TL, 1, K_DCT2C, IO_LINE, IO_LINE>(v);
-  add<R, M, TL, 1, K_DCT3C, IO_LINE, IO_LINE>(v);
-  add<R, M, TT, 1, K_DCT2C, IO_LINE, IO_TILE>(v);
-  add<R, M, TT, 2, K_DCT2C, IO_LINE, IO_TILE>(v);
-  add<R, M, TT, 1, K_DCT3C, IO_LINE, IO_TILE>(v);
-  add<R, M, TT, 2, K_DCT3C, IO_LINE, IO_TILE>(v);
-  add<R, M, TT, 1, K_DCT2C, IO_TILE, IO_LINE>(v);
-  add<R, M, TT, 1, K_DCT3C, IO_TILE, IO_LINE>(v);
+  add<R, M, TL, 1, K, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K, IO_LINE, IO_TILE>(v);
+  if constexpr (MORE) add<R, M, TT, 2, K, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K, IO_TILE, IO_LINE>(v);
+}
+
+template <typename R, int M>
+static void add_dct(std::vector<Entry>& v) {
+  add_r2r_kind<R, M, K_DCT2C, true>(v);
+  add_r2r_kind<R, M, K_DCT3C, true>(v);
+  add_r2r_kind<R, M, K_DCT2C | K_REAL_PAIRS, true>(v);
+  add_r2r_kind<R, M, K_DCT3C | K_REAL_PAIRS, true>(v);
+  add_r2r_kind<R, M, K_DST2C, false>(v);
+  add_r2r_kind<R, M, K_DST3C, false>(v);
+  add_r2r_kind<R, M, K_DHTC, false>(v);
+  add_r2r_kind<R, M, K_DST2C | K_REAL_PAIRS, false>(v);
+  add_r2r_kind<R, M, K_DST3C | K_REAL_PAIRS, false>(v);
+  add_r2r_kind<R, M, K_DHTC | K_REAL_PAIRS, false>(v);
 }
 
 template <typename R, int M>
@@ -1262,6 +1527,7 @@ static const std::vector<Entry>& registry() {
     add_size<float, 256>(r);
     add_size<float, 512>(r);
     add_size<float, 1024>(r);
+    add_mfma32(r);
     add_mfma<1>(r);
     add_mfma<2>(r);
     add_size<float, 32>(r);
@@ -1355,17 +1621,56 @@ static bool all_even(const int64_t* s, int from, int D) {
 }
 
 // Specialised kernels that can run a pass (first = default choice).
+// The arguments of an R2R pass on real data seen as complex lines, each made
+// of the real lines (2j, 2j + 1) along line dim 0 (K_REAL_PAIRS).  On a side
+// where the lines are contiguous (element stride 1) the pair's lines stay
+// apart: the side keeps real units with the pair stride 2 ls[0], and the
+// kernel finds the second line ls[0] / 2 after the first.  On a transposed
+// side the two lines are adjacent (ls[0] = 1), so one complex element holds
+// both, and the side is addressed in complex units.  False where the pairs
+// do not fit the layout.
+static bool pair_view(PassArgs& a) {
+  if (a.ldim[0] % 2) return false;
+  const bool in_line = a.sk_in == 1, out_line = a.sk_out == 1;
+  if ((!in_line && a.lsin[0] != 1) || (!out_line && a.lsout[0] != 1)) return false;
+  auto side = [&](bool line, int64_t* ls, int64_t* sk) {
+    if (line) {
+      ls[0] *= 2;
+      return true;
+    }
+    if (*sk % 2) return false;
+    for (int d = 1; d < a.D; ++d)
+      if (ls[d] % 2) return false;
+    *sk /= 2;
+    for (int d = 1; d < a.D; ++d) ls[d] /= 2;
+    return true;
+  };
+  if (!side(in_line, a.lsin, &a.sk_in) || !side(out_line, a.lsout, &a.sk_out)) return false;
+  a.ldim[0] /= 2;
+  return true;
+}
+
 std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
   std::vector<int> out;
   if (p.flags & PFFT_FLAG_GENERIC_ONLY) return out;
-  const PassArgs& a = pp.args;
+  PassArgs a = pp.args;
   int kind;
-  if (a.kind == L_C2C) kind = K_C2C;
-  else if (a.kind == L_R2C_HALF) kind = K_R2C;
-  else if (a.kind == L_C2R_HALF) kind = K_C2R;
-  else if (a.kind == L_DCT2 && a.r2r_cplx) kind = K_DCT2C;
-  else if (a.kind == L_DCT3 && a.r2r_cplx) kind = K_DCT3C;
-  else return out;
+  switch (a.kind) {
+    case L_C2C: kind = K_C2C; break;
+    case L_R2C_HALF: kind = K_R2C; break;
+    case L_C2R_HALF: kind = K_C2R; break;
+    case L_DCT2: kind = K_DCT2C; break;
+    case L_DCT3: kind = K_DCT3C; break;
+    case L_DST2: kind = K_DST2C; break;
+    case L_DST3: kind = K_DST3C; break;
+    case L_DHT: kind = K_DHTC; break;
+    default: return out;
+  }
+  // R2R of real data: pairs of lines as complex lines
+  if (kind >= K_DCT2C && !a.r2r_cplx) {
+    if (!pair_view(a)) return out;
+    kind |= K_REAL_PAIRS;
+  }
   const int rb = p.real_type == PFFT_F32 ? 4 : 8;
   const int M = (int)a.M;
   int ld, st;
@@ -1380,16 +1685,17 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
   if (kind == K_R2C && !all_even(a.lsin, 0, a.D)) return out;
   if (kind == K_C2R && !all_even(a.lsout, 0, a.D)) return out;
   // precision flags put the radix-16 stages of 1024-point c2c lines on MFMA
-  int want_mf = 0;
+  int want_mf = 0, alt_mf = -1;
   if (kind == K_C2C && M == 1024 && rb == 4) {
-    if (p.flags & PFFT_FLAG_MFMA_SPLIT16) want_mf = 1;
+    // split-FP16: the 32 x 32 register form (first) and the radix-16 LDS form
+    if (p.flags & PFFT_FLAG_MFMA_SPLIT16) want_mf = 3, alt_mf = 1;
     else if (p.flags & PFFT_FLAG_MFMA_BF16X3) want_mf = 2;
   }
   const std::vector<Entry>& reg = registry();
   for (size_t i = 0; i < reg.size(); ++i) {
     const Entry& x = reg[i];
     if (x.real_bytes != rb || x.M != M || x.kind != kind || x.ld != ld || x.st != st) continue;
-    if (x.mf != want_mf) continue;
+    if (x.mf != want_mf && x.mf != alt_mf) continue;
     if (x.opk != (op ? 1 : 0)) continue;
     if (x.lds > 160 * 1024) continue;
     if (ld == IO_TILE || st == IO_TILE) {
@@ -1416,9 +1722,13 @@ void fast_apply(PassPlan& pp, int variant) {
   PassArgs& a = pp.args;
   a.tile_order = (variant & kOrderBit) ? 1 : 0;
   a.tile_rev = (variant & kVariantReverse) ? 1 : 0;
+  // real data of an R2R pass as line pairs (the candidates checked it fits)
+  const bool rp = (e.kind & K_REAL_PAIRS) != 0;
+  if (rp) pair_view(a);
   // contiguous passes: keep the input-adjacent line dim first (its partial
-  // last tile costs little) unless it is shorter than a tile
-  if (e.ld == IO_LINE && e.st == IO_LINE && a.ldim[0] < e.T) {
+  // last tile costs little) unless it is shorter than a tile (not for line
+  // pairs, which run along line dim 0)
+  if (!rp && e.ld == IO_LINE && e.st == IO_LINE && a.ldim[0] < e.T) {
     for (int d = 1; d < a.D; ++d) {
       if (a.ldim[d] % e.T == 0) {
         std::swap(a.ldim[0], a.ldim[d]);
@@ -1478,9 +1788,13 @@ const char* fast_name(int variant) {
   const int idx = variant_index(variant);
   if (variant < 1 || idx < 0 || idx >= (int)reg.size()) return "generic";
   const Entry& e = reg[idx];
-  static const char* kinds[5] = {"c2c", "r2c", "c2r", "dct2c", "dct3c"};
+  // R2R kinds end in c (complex data) or r (real data, line pairs)
+  static const char* kinds[8] = {"c2c", "r2c", "c2r", "dct2", "dct3", "dst2", "dst3", "dht"};
+  const int kb = e.kind & 15;
+  const std::string kname =
+      std::string(kinds[kb]) + (kb >= K_DCT2C ? ((e.kind & K_REAL_PAIRS) ? "r" : "c") : "");
   s = std::string("k_fast<") + (e.real_bytes == 4 ? "f32" : "f64") + "," +
-      std::to_string(e.M) + "," + kinds[e.kind] + "," + (e.ld ? "tile" : "line") + "->" +
+      std::to_string(e.M) + "," + kname + "," + (e.ld ? "tile" : "line") + "->" +
       (e.st ? "tile" : "line") + ",T=" + std::to_string(e.T) + ",LPW=" +
       std::to_string(e.lpw) + (e.nsp > 1 ? ",S=" + std::to_string(e.nsp) : std::string()) +
       (e.w > 1 ? ",W=" + std::to_string(e.w) : std::string()) +
@@ -1488,9 +1802,29 @@ const char* fast_name(int variant) {
       (e.opk ? ",op" : "") + (e.pf ? ",PF" : "") +
       ((variant & kOrderBit) ? ",ord1" : "") + ((variant & kVariantReverse) ? ",rev" : "") +
       ((variant & kVariantStream) ? ",nt" : "") +
-      (e.mf == 1 ? ",mfma-split16" : e.mf == 2 ? ",mfma-bf16x3" : "") +
+      (e.mf == 1   ? ",mfma-split16"
+       : e.mf == 2 ? ",mfma-bf16x3"
+       : e.mf == 3 ? ",mfma32-split16"
+                   : "") +
       ">";
   return s.c_str();
+}
+
+std::string fast_build_signature() {
+  const std::vector<Entry>& reg = registry();
+  uint64_t h = 1469598103934665603ull;   // FNV-1a, 64 bit
+  for (size_t i = 0; i < reg.size(); ++i) {
+    const std::string n = fast_name((int)i + 1);
+    for (char ch : n) {
+      h ^= (uint8_t)ch;
+      h *= 1099511628211ull;
+    }
+    h ^= 0xff;
+    h *= 1099511628211ull;
+  }
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%zu-%016llx", reg.size(), (unsigned long long)h);
+  return buf;
 }
 
 #endif  // PFFT_FAST_NT

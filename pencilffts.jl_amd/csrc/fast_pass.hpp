@@ -18,5 +18,8 @@ void fast_apply(PassPlan& pp, int variant);
 int fast_prepare(const PassPlan& pp);
 int fast_launch(const PassPlan& pp, const PassArgs& a, hipStream_t stream);
 const char* fast_name(int variant);  // kernel name of a variant ("generic" for 0)
+// Digest of this build's kernel table (names of every entry, in order): tune
+// data names kernels by table index, so it is valid for one table only.
+std::string fast_build_signature();
 
 }  // namespace pfft

@@ -65,8 +65,11 @@ static int set_lds_limit(const void* fn, int64_t bytes) {
 // A pass can usually run on several specialised variants (tile shape, lines
 // per wave).  The first use of a pass shape times every candidate on scratch
 // buffers of the pass's size and keeps the fastest; the choice is cached per
-// process by the pass signature.  Variants differ only in how work is spread,
-// not in arithmetic, so results do not depend on the choice.
+// process by the pass signature and its place in the plan.  Within one line
+// length the variants differ only in how work is spread, so their results
+// are bit-identical; the exception is 1024-point lines, whose two-wavefront
+// variants (W = 2) run the 8.8.4.4 radix schedule instead of 16.16.4 and
+// agree with the others to rounding only.
 static std::mutex g_tune_mu;
 static std::map<std::string, int> g_tune_cache;
 
@@ -80,6 +83,19 @@ static std::string pass_key(const Plan& p, const PassPlan& pp, const std::vector
     << ':' << a.sk_in << ':' << a.sk_out << ':' << a.in_cplx << a.out_cplx << ':' << a.D;
   for (int d = 0; d < a.D; ++d) o << ':' << a.ldim[d] << ',' << a.lsin[d] << ',' << a.lsout[d];
   return o.str();
+}
+
+// Signature of a whole plan for the tune cache: every pass of both
+// directions, in-place or not, and whether the in-chain phases run.
+static std::string plan_signature(const Plan& p, bool chain) {
+  std::string s = std::string(p.inplace ? "ip" : "oop") + (chain ? ",chain" : ",pass");
+  const std::vector<int> none;
+  for (const std::vector<PassPlan>* list : {&p.fwd, &p.bwd}) {
+    s += "[";
+    for (const PassPlan& pp : *list) s += pass_key(p, pp, none) + ";";
+    s += "]";
+  }
+  return s;
 }
 
 // op_side > 0 times an operator-carrying variant with a derivative operator
@@ -190,11 +206,13 @@ static int prepare_pass(const Plan& p, const PassPlan& pp) {
 // kernels are those of the pipeline, and a choice that speeds its own pass
 // but slows the next one loses.  Out-of-place plans; skipped when the buffers
 // do not fit.
-static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
+static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log, bool* ran) {
+  *ran = true;
   if (p.inplace) return PFFT_OK;
   bool any = false;
   for (const TuneJob& j : jobs) any = any || j.op_side == 0;
   if (!any) return PFFT_OK;
+  *ran = false;
   void *u = nullptr, *v = nullptr, *w = nullptr;
   hipStream_t st = nullptr;
   const int kPairs = 5;
@@ -226,6 +244,7 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
     (void)hipGetLastError();
     return PFFT_OK;
   }
+  *ran = true;
   (void)hipMemsetD32Async((hipDeviceptr_t)u, 0x3f800000u, p.in_bytes / 4, st);
   void* bufs[3] = {nullptr, w, w ? (char*)w + p.work0_bytes : nullptr};
   // runs `pairs` mul! + ldiv! pairs, recording events around pair i
@@ -335,6 +354,7 @@ static int refine_in_chain(Plan& p, std::vector<TuneJob>& jobs, bool log) {
                 c == best ? "  <- chosen" : "");
     }
     rc = apply(cands[best]);
+    if (rc) break;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[j.key] = cands[best];
   }
@@ -416,6 +436,10 @@ static int autotune(Plan& p) {
   std::vector<Job> jobs;
   size_t in_max = 0, out_max = 0, prod_max = 0;
   const uint32_t forced = (p.flags >> 16) & 0xffu;
+  // PFFT_TUNE_CHAIN=0 keeps the first phase's choices
+  const char* chain_env = getenv("PFFT_TUNE_CHAIN");
+  const bool chain = !(chain_env && chain_env[0] == '0') && !p.inplace;
+  const std::string psig = plan_signature(p, chain);
   auto settle = [&](PassPlan& pp, int op_side, int v) {
     if (op_side) pp.variant_op = v;
     else fast_apply(pp, v);
@@ -430,18 +454,32 @@ static int autotune(Plan& p) {
       settle(pp, op_side, c[(forced - 1) % c.size()] | rev);
       return;
     }
-    // the chain position is part of the key: the in-chain phases choose with
-    // the pass's neighbours in place, so two passes of one signature may differ
-    const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "") +
-                            "@" + std::to_string(list) + "." + std::to_string(index) + "/" +
-                            std::to_string(list == 0 ? p.fwd.size() : p.bwd.size());
-    auto it = g_tune_cache.find(key);
-    if (it != g_tune_cache.end()) {
-      settle(pp, op_side, it->second);
-      return;
-    }
+    // untuned plans take the first candidate whatever a tuned plan of the
+    // same shape chose before (the pinned, reproducible choice)
     if (c.size() == 1 || (p.flags & PFFT_FLAG_NO_AUTOTUNE)) {
       settle(pp, op_side, c[0] | rev);
+      return;
+    }
+    // The key holds the pass signature, its position in the chain and the
+    // signature of the whole plan: the in-chain phases choose with the
+    // pass's neighbours in place, so the same pass in another chain (or in a
+    // chain tuned without them) may choose differently.
+    const std::string key = pass_key(p, base, c) + (op_side ? ":op" + std::to_string(op_side) : "") +
+                            "@" + std::to_string(list) + "." + std::to_string(index) + "|" + psig;
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      // a cached choice (imported ones included) must be one of this pass's
+      // candidates, up to the direction and streaming bits
+      const int base_v = it->second & ~(kVariantReverse | kVariantStream);
+      if (std::find(c.begin(), c.end(), base_v) != c.end()) {
+        settle(pp, op_side, it->second);
+        return;
+      }
+    }
+    // estimate plans take what earlier tuning (or imported tune data) chose
+    // for this pass, else the first candidate, and time nothing
+    if (p.flags & PFFT_FLAG_ESTIMATE) {
+      settle(pp, op_side, c[0]);
       return;
     }
     jobs.push_back({&pp, base, c, key, op_side, prod, list, index, {}});
@@ -558,37 +596,107 @@ static int autotune(Plan& p) {
     if (rc) break;
   }
   cleanup();
-  // PFFT_TUNE_CHAIN=0 keeps the first phase's choices
-  const char* chain_env = getenv("PFFT_TUNE_CHAIN");
-  if (!rc && !(chain_env && chain_env[0] == '0')) rc = refine_in_chain(p, jobs, tune_log);
+  if (!rc && chain) {
+    bool ran = false;
+    rc = refine_in_chain(p, jobs, tune_log, &ran);
+    if (!ran) {
+      // the in-chain phases did not fit in device memory: these phase-1
+      // choices are not what a chain-tuned plan of this signature would
+      // take, so they are not kept for later plans
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      for (const Job& j : jobs) g_tune_cache.erase(j.key);
+    }
+  }
   return rc;
+}
+
+// ---- tune data export / import ------------------------------------------------
+// The tune cache as text: a header naming the format, the kernel table of
+// this build and the device, then one "variant<TAB>key" line per choice.
+// Variant ids index this build's kernel table, so data from another build
+// or another kind of device is refused.
+static std::string device_tag() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return "none";
+  }
+  return std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+}
+
+static std::string tune_header() {
+  return "pfft-tune 1\nbuild " + fast_build_signature() + "\ndevice " + device_tag() + "\n";
+}
+
+int tune_export(std::string* out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  *out = tune_header();
+  for (const auto& kv : g_tune_cache) *out += std::to_string(kv.second) + "\t" + kv.first + "\n";
+  return PFFT_OK;
+}
+
+int tune_import(const char* text) {
+  const std::string s(text);
+  const std::string head = tune_header();
+  if (s.compare(0, head.size(), head) != 0)
+    return fail(PFFT_ERR_ARGUMENT,
+                "tune data of another format, build or device (expected a header of\n" + head +
+                    ")");
+  std::map<std::string, int> got;
+  size_t pos = head.size();
+  while (pos < s.size()) {
+    size_t nl = s.find('\n', pos);
+    if (nl == std::string::npos) nl = s.size();
+    const std::string line = s.substr(pos, nl - pos);
+    pos = nl + 1;
+    if (line.empty()) continue;
+    const size_t tab = line.find('\t');
+    char* end = nullptr;
+    const long v = tab == std::string::npos ? 0 : strtol(line.c_str(), &end, 10);
+    if (tab == std::string::npos || end != line.c_str() + tab || v < 0 || tab + 1 >= line.size())
+      return fail(PFFT_ERR_ARGUMENT, "malformed tune data line: " + line);
+    got[line.substr(tab + 1)] = (int)v;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const auto& kv : got) g_tune_cache[kv.first] = kv.second;
+  return PFFT_OK;
+}
+
+void tune_forget() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache.clear();
 }
 
 int device_prepare(Plan& p) {
   if (p.dev) return PFFT_OK;
   DeviceState* d = new DeviceState();
-  if (hipGetDevice(&d->device) != hipSuccess) {
+  // on failure the tables go too, so a later prepare starts clean
+  auto abandon = [&](int code) {
     delete d;
-    return fail(PFFT_ERR_DEVICE, "no HIP device available");
-  }
+    for (Table& t : p.tables) {
+      if (t.dev) (void)hipFree(t.dev);
+      t.dev = nullptr;
+    }
+    return code;
+  };
+  if (hipGetDevice(&d->device) != hipSuccess)
+    return abandon(fail(PFFT_ERR_DEVICE, "no HIP device available"));
   for (Table& t : p.tables) {
     const size_t bytes = t.host.size();
     hipError_t e = hipMalloc(&t.dev, bytes);
     if (e != hipSuccess) {
-      delete d;
-      return fail(PFFT_ERR_ALLOC, std::string("hipMalloc(tables): ") + hipGetErrorString(e));
+      t.dev = nullptr;
+      return abandon(
+          fail(PFFT_ERR_ALLOC, std::string("hipMalloc(tables): ") + hipGetErrorString(e)));
     }
     e = hipMemcpy(t.dev, t.host.data(), bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      delete d;
-      return fail(PFFT_ERR_DEVICE, std::string("hipMemcpy(tables): ") + hipGetErrorString(e));
-    }
+    if (e != hipSuccess)
+      return abandon(
+          fail(PFFT_ERR_DEVICE, std::string("hipMemcpy(tables): ") + hipGetErrorString(e)));
   }
   int r = autotune(p);
-  if (r) {
-    delete d;
-    return r;
-  }
+  if (r) return abandon(r);
   for (auto* list : {&p.fwd, &p.bwd}) {
     for (PassPlan& pp : *list) {
       if (pp.variant == 0) {
@@ -597,10 +705,7 @@ int device_prepare(Plan& p) {
       } else {
         r = fast_prepare(pp);
       }
-      if (r) {
-        delete d;
-        return r;
-      }
+      if (r) return abandon(r);
     }
   }
   p.dev = d;

@@ -415,6 +415,17 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
       for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
       for (int64_t k = 0; k < n; ++k)
         expm(2 * k + 1, 8 * n, &aux[2 * (n + k)], &aux[2 * (n + k) + 1]);
+    } else if (lk == L_C2C && M == 1024 && p.real_type == PFFT_F32) {
+      // middle twiddles of the 32 x 32 MFMA form of 1024-point lines, in the
+      // order its lanes read them: entry 64 i + l is w1024^(k2 n1) with
+      // k2 = l mod 32 and n1 = (i mod 4) + 8 (i div 4) + 4 (l div 32), the
+      // row of accumulator register i of lane l
+      aux.resize(2 * 1024);
+      for (int i = 0; i < 16; ++i)
+        for (int l = 0; l < 64; ++l) {
+          const int64_t k2 = l & 31, n1 = (i & 3) + 8 * (i >> 2) + 4 * (l >> 5);
+          expm(k2 * n1, 1024, &aux[2 * (64 * i + l)], &aux[2 * (64 * i + l) + 1]);
+        }
     }
     if (!aux.empty()) pp.aux_id = add_table(p, "aux", std::move(aux));
   }

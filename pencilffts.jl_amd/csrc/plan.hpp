@@ -121,6 +121,10 @@ int device_set_timing(Plan& p, bool on);
 int device_project(Plan& p, void* const* uh, const double* scale, hipStream_t stream);
 int device_stage_times(Plan& p, int n, float* ms);
 int device_pass_names(Plan& p, int dir, std::string* out);
+// tune cache as text (pfft_tune_export / pfft_tune_import / pfft_tune_forget)
+int tune_export(std::string* out);
+int tune_import(const char* text);
+void tune_forget();
 size_t eltype_bytes(int32_t eltype);
 
 }  // namespace pfft

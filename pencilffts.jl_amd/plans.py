@@ -160,8 +160,13 @@ class PencilFFTPlan:
 
     `transforms` is one transform (expanded with expand_dims, plans.jl:316-320)
     or one per dimension.  proc_dims must describe a single process (one GPU);
-    fftw_flags, fftw_timelimit and transpose_method are accepted and ignored,
-    as the reference does for GPU arrays (plans.jl:325-332).
+    fftw_timelimit and transpose_method are accepted and ignored, as the
+    reference does for GPU arrays (plans.jl:325-332).  fftw_flags chooses
+    how kernels are picked, as FFTW's planner flags do (plans.jl:241-250):
+    FFTW_ESTIMATE (or FFTW_WISDOM_ONLY) times nothing and takes tuned
+    choices already known to the process (see import_wisdom), else the
+    first candidate of each pass; FFTW_MEASURE / PATIENT / EXHAUSTIVE, and
+    None (not given), tune the plan on its first execute.
     """
 
     def __init__(self, dims_global, transforms, proc_dims=None, comm=None, real_type=None, *,
@@ -199,6 +204,10 @@ class PencilFFTPlan:
         self.extra_dims = tuple(int(e) for e in extra_dims)
         self.timer = timer
         self.device = device
+        self.fftw_flags = fftw_flags
+        flags = int(flags)
+        if fftw_flags is not None and int(fftw_flags) & (L.FFTW_ESTIMATE | L.FFTW_WISDOM_ONLY):
+            flags |= L.FLAG_ESTIMATE
         h = C.c_void_p()
         dims_arr = (C.c_int64 * N)(*dims_global)
         kinds = (C.c_int32 * N)(*[t.code for t in transforms])
@@ -475,6 +484,35 @@ class Val:
 
 def scale_factor(p):
     return p.scale_factor
+
+
+def export_wisdom(path=None):
+    """The tuned kernel choices of this process as text (pfft_tune_export),
+    written to `path` when given: the counterpart of FFTW.export_wisdom."""
+    need = C.c_size_t()
+    L.check(L.lib.pfft_tune_export(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    L.check(L.lib.pfft_tune_export(buf, need.value, C.byref(need)))
+    text = buf.value.decode()
+    if path is not None:
+        with open(path, "w") as f:
+            f.write(text)
+    return text
+
+
+def import_wisdom(path_or_text):
+    """Merge tuned choices exported by export_wisdom (a file path or the text
+    itself) into this process (pfft_tune_import): FFTW.import_wisdom."""
+    text = path_or_text
+    if not text.startswith("pfft-tune"):
+        with open(path_or_text) as f:
+            text = f.read()
+    L.check(L.lib.pfft_tune_import(text.encode()))
+
+
+def forget_wisdom():
+    """Drop every tuned choice of this process: FFTW.forget_wisdom."""
+    L.check(L.lib.pfft_tune_forget())
 
 
 def execute_many(direction, p, dsts, srcs):

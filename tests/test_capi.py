@@ -220,3 +220,23 @@ def test_execute_argument_checks_without_gpu(pf):
     # is length(dst) ≠ length(src), as check_compatible(dst, src) formats it
     with pytest.raises(pf.ArgumentError, match="collections have different lengths: 2 ≠ 3"):
         pf.mul_([1, 2], plan, [1, 2, 3])
+
+
+def test_tune_data_round_trip_and_refusal(pf):
+    """pfft_tune_export / pfft_tune_import on the host: the export carries the
+    build and device header, imports back, and data of another build, a bad
+    line or a foreign format is refused with ArgumentError."""
+    text = pf.export_wisdom()
+    head = text.split("\n")[:3]
+    assert head[0] == "pfft-tune 1" and head[1].startswith("build ") and \
+        head[2].startswith("device ")
+    pf.import_wisdom(text)
+    assert pf.export_wisdom() == text
+    with pytest.raises(pf.ArgumentError):
+        pf.import_wisdom("pfft-tune 1\nbuild 0-0000000000000000\ndevice none\n")
+    with pytest.raises(pf.ArgumentError):
+        pf.import_wisdom("\n".join(head) + "\nnot-a-number\tkey\n")
+    pf.import_wisdom("\n".join(head) + "\n7\tsome:key\n")
+    assert "7\tsome:key" in pf.export_wisdom()
+    pf.forget_wisdom()
+    assert pf.export_wisdom() == "\n".join(head) + "\n"

@@ -1,0 +1,90 @@
+"""The nine R2R kinds of the reference (src/Transforms/r2r.jl:5-15, planned at
+:86-93) on the GPU against the FP64 oracle, at the lengths the specialised
+passes cover: real data ((R2R(kind),) x 3: every pass an R2R of real lines)
+and complex data ((FFT, R2R(kind), R2R(kind))).  REDFT10 / REDFT01 /
+RODFT10 / RODFT01 / DHT run on the specialised kernels (pre/post-twiddled
+FFTs; real data two lines per complex FFT), the others on the generic pass;
+each case records which.  Every specialised candidate is also forced once."""
+import numpy as np
+import pytest
+
+from tests.test_gpu_parity import run_case
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+KINDS = ["REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01", "RODFT10",
+         "RODFT11", "DHT"]
+FAST = {"REDFT01", "REDFT10", "RODFT01", "RODFT10", "DHT"}
+
+
+def _trs(T, kind, data):
+    k = getattr(T, kind)
+    if data == "real":
+        return (T.R2R(k),)
+    return (T.FFT(), T.R2R(k), T.R2R(k))
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["F32", "F64"])
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("kind", KINDS)
+def test_r2r_kinds(pf, gpu, kind, data, real_type):
+    T = pf.Transforms
+    dims = (256, 64, 128) if real_type == 0 else (256, 64, 64)
+    err_f, err_b, meta = run_case(pf, dims, _trs(T, kind, data), real_type, seed=11)
+    n = int(np.prod(dims))
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
+    print(kind, data, real_type, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= tol and err_b <= tol
+    if kind in FAST:
+        # every R2R pass on the specialised kernels
+        assert all(k.startswith("k_fast") for k in meta.kernels), meta.kernels
+
+
+def test_r2r_real_chebyshev_512(pf, gpu):
+    """A pure Chebyshev plan, (R2R(REDFT10),) x 3 on a Float32 512^3 field, runs
+    only specialised passes; its error against the oracle and round trip."""
+    from oracle import pipeline as OP
+    from oracle import plan as OM
+    T = pf.Transforms
+    dims = (512, 512, 512)
+    plan = pf.PencilFFTPlan(dims, T.R2R(T.REDFT10), (1, 1), None, T.Float32)
+    ks = plan.kernels(0) + plan.kernels(2)
+    print(ks)
+    assert all(k.startswith("k_fast") for k in ks), ks
+    u = pf.allocate_input(plan)
+    u.parent.uniform_(-1, 1, generator=torch.Generator(device="cuda").manual_seed(3))
+    v = plan * u
+    w = plan.ldiv(v)
+    torch.cuda.synchronize()
+    x = u.parent.cpu().numpy().astype(np.float64).reshape(-1)
+    err_b = OP.rel_l2(w.parent.cpu().numpy().reshape(-1), x)
+    meta = OM.PlanMeta(dims, ("R2R_REDFT10",), "Float64")
+    ref = OP.forward(meta, OP.from_image(x, dims), workers=16)
+    err_f = OP.rel_l2(v.parent.cpu().numpy().reshape(-1), OP.memory_image(ref))
+    print("512^3 REDFT10 x3: fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= 4.5e-6 and err_b <= 4.5e-6
+
+
+MAX_CANDIDATES = 16
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["F32", "F64"])
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("kind", sorted(FAST))
+def test_r2r_every_candidate(pf, gpu, kind, data, real_type):
+    T = pf.Transforms
+    L = pf._lib
+    dims = (128, 64, 64) if real_type == 0 else (64, 64, 64)
+    n = int(np.prod(dims))
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(n)
+    seen = set()
+    for k in range(MAX_CANDIDATES):
+        rev = L.FLAG_REVERSE_TILES | L.FLAG_STREAM_STORES if k % 2 else 0
+        err_f, err_b, meta = run_case(pf, dims, _trs(T, kind, data), real_type, seed=k,
+                                      flags=L.FLAG_CANDIDATE(k) | rev)
+        seen.update(meta.kernels)
+        assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
+    print(kind, data, real_type, sorted(seen))
+    tag = "r," if data == "real" else "c,"
+    assert any(tag in s and ("dct" in s or "dst" in s or "dht" in s) for s in seen), seen

@@ -1,0 +1,107 @@
+"""Kernel choice: pinned (PFFT_FLAG_NO_AUTOTUNE) plans after tuned ones, and
+tune data carried to a second process (pfft_tune_export / pfft_tune_import,
+the counterpart of FFTW wisdom) so that it plans without timing anything and
+computes bit-identical results."""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import time
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _digest(t):
+    return hashlib.sha256(t.detach().cpu().contiguous().numpy().tobytes()).hexdigest()
+
+
+def test_no_autotune_after_tuned_plan(pf, gpu):
+    """A NO_AUTOTUNE plan built after a tuned plan of the same shape takes
+    the first candidate of every pass, not the tuned choice."""
+    T = pf.Transforms
+    L = pf._lib
+    dims = (128, 128, 64)
+    tuned = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float32)
+    tuned.prepare()
+    pinned = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float32,
+                              flags=L.FLAG_NO_AUTOTUNE)
+    first = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float32,
+                             flags=L.FLAG_CANDIDATE(0))
+    for d in (L.FORWARD, L.BACKWARD):
+        assert pinned.kernels(d) == first.kernels(d), (pinned.kernels(d), first.kernels(d))
+    print("tuned", tuned.kernels(0) + tuned.kernels(2))
+    print("pinned", pinned.kernels(0) + pinned.kernels(2))
+
+
+CHILD = r"""
+import json, sys, time, hashlib
+sys.path.insert(0, %(root)r)
+import torch
+import __graft_entry__ as ge
+pf = ge.load_package()
+T = pf.Transforms
+torch.zeros(1, device="cuda"); torch.cuda.synchronize()
+pf.import_wisdom(%(path)r)
+# the process's first prepared plan also loads the library's code object
+# onto the device (once per process): a small plan takes that cost here
+t0 = time.perf_counter()
+p16 = pf.PencilFFTPlan((16, 16, 16), T.RFFT(), (1, 1), None, T.Float32)
+pf._lib.check(pf._lib.lib.pfft_plan_prepare(p16._h, None))
+t_first = time.perf_counter() - t0
+# planning: pfft_plan_create + pfft_plan_prepare (tables uploaded, kernels
+# chosen); the workspace is the caller's memory and is allocated apart
+t0 = time.perf_counter()
+plan = pf.PencilFFTPlan(%(dims)r, T.RFFT(), (1, 1), None, T.Float32,
+                        fftw_flags=pf._lib.FFTW_ESTIMATE if %(estimate)r else None)
+pf._lib.check(pf._lib.lib.pfft_plan_prepare(plan._h, None))
+t_plan = time.perf_counter() - t0
+plan.prepare()
+u = pf.allocate_input(plan)
+u.parent.uniform_(-1, 1, generator=torch.Generator(device="cuda").manual_seed(21))
+v = plan * u
+torch.cuda.synchronize()
+h = hashlib.sha256(v.parent.cpu().contiguous().numpy().tobytes()).hexdigest()
+print("RESULT " + json.dumps({"t_plan": t_plan, "t_first": t_first,
+                              "kernels": plan.kernels(0) + plan.kernels(2),
+                              "digest": h}))
+"""
+
+
+@pytest.mark.parametrize("estimate", [False, True], ids=["default", "estimate"])
+def test_wisdom_in_second_process(pf, gpu, tmp_path, estimate):
+    """512^3 Float32 RFFT: a second process that imports this one's tune data
+    plans in under 50 ms, with the same kernels and bit-identical output."""
+    T = pf.Transforms
+    dims = (512, 512, 512)
+    plan = pf.PencilFFTPlan(dims, T.RFFT(), (1, 1), None, T.Float32)
+    t0 = time.perf_counter()
+    plan.prepare()
+    t_tune = time.perf_counter() - t0
+    kernels = plan.kernels(0) + plan.kernels(2)
+    u = pf.allocate_input(plan)
+    u.parent.uniform_(-1, 1, generator=torch.Generator(device="cuda").manual_seed(21))
+    v = plan * u
+    torch.cuda.synchronize()
+    digest = _digest(v.parent)
+    del plan, u, v
+    torch.cuda.empty_cache()
+    path = str(tmp_path / "pfft.wisdom")
+    text = pf.export_wisdom(path)
+    assert text.startswith("pfft-tune 1\n") and text.count("\n") > 3
+    src = CHILD % {"root": ROOT, "path": path, "dims": dims, "estimate": estimate}
+    r = subprocess.run([sys.executable, "-c", src], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT ")][-1]
+    res = json.loads(line[len("RESULT "):])
+    print("first process: tuning %.3f s; second process: planning %.4f s (its first, "
+          "16^3 plan, with the code object load: %.4f s)" % (t_tune, res["t_plan"],
+                                                            res["t_first"]), res["kernels"])
+    assert res["kernels"] == kernels
+    assert res["digest"] == digest
+    assert res["t_plan"] < 0.05, res["t_plan"]
