@@ -200,6 +200,23 @@ int pfft_plan_describe(const pfft_plan* plan, char* buf, size_t buflen);
 int pfft_execute(pfft_plan* plan, int32_t direction, const void* in, void* out,
                  void* work, size_t work_bytes, void* hip_stream);
 
+/* pfft_execute that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot call hipMalloc -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation.  The
+ * intermediate pencils go to a static device array of the library
+ * (PFFT_NOALLOC_SCRATCH_BYTES, 256 MiB, held on every GPU the library runs
+ * on until the process ends) instead of a workspace; the caller passes the
+ * input and output arrays as for pfft_execute.  The plan must be prepared
+ * (pfft_plan_prepare: tables are uploaded and kernels tuned there, which
+ * allocates), and its work_bytes (pfft_plan_nbytes) must not exceed the
+ * scratch size; either failure returns PFFT_ERR_ARGUMENT.  Spectral
+ * operators are not available here.  All calls on one GPU share its array,
+ * and its contents are whatever the last call left: two calls that overlap
+ * in time on one GPU (other streams or threads) race.                      */
+#define PFFT_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int pfft_execute_noalloc(pfft_plan* plan, int32_t direction, const void* in, void* out,
+                         void* hip_stream);
+
 /* Many arrays through one plan: the reference broadcasts mul!/ldiv!/bmul!
  * over tuples or arrays of PencilArrays one at a time (operations.jl:139-152,
  * with the length check of :130-137).  Here all n_arrays transforms are

@@ -18,6 +18,7 @@ FLAG_REVERSE_TILES = 16
 FLAG_STREAM_STORES = 32
 FLAG_DENSE_WORK = 64
 FLAG_ESTIMATE = 128
+NOALLOC_SCRATCH_BYTES = 256 << 20   # static scratch of pfft_execute_noalloc
 
 # FFTW planner flags (FFTW.jl's constants), accepted by PencilFFTPlan
 FFTW_MEASURE, FFTW_EXHAUSTIVE, FFTW_PATIENT, FFTW_ESTIMATE, FFTW_WISDOM_ONLY = (
@@ -69,6 +70,7 @@ SIGNATURES = {
     "pfft_plan_stage": (C.c_int, [vp, i32, P(i32), P(i64), P(i32), P(i64), P(i32)]),
     "pfft_plan_describe": (C.c_int, [vp, C.c_char_p, sz]),
     "pfft_execute": (C.c_int, [vp, i32, vp, vp, vp, sz, vp]),
+    "pfft_execute_noalloc": (C.c_int, [vp, i32, vp, vp, vp]),
     "pfft_execute_op": (C.c_int, [vp, i32, C.POINTER(SpectralOp), vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
     "pfft_project_divergence_free": (C.c_int, [vp, vp, vp, vp, P(C.c_double), vp]),

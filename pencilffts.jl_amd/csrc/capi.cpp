@@ -261,6 +261,13 @@ int pfft_execute(pfft_plan* plan, int32_t direction, const void* in, void* out, 
                                    (hipStream_t)hip_stream);)
 }
 
+int pfft_execute_noalloc(pfft_plan* plan, int32_t direction, const void* in, void* out,
+                         void* hip_stream) {
+  NEED(plan);
+  PFFT_GUARD(return device_execute_noalloc(plan->p, direction, in, out,
+                                           (hipStream_t)hip_stream);)
+}
+
 int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* op,
                     const void* in, void* out, void* work, size_t work_bytes, void* hip_stream) {
   NEED(plan);

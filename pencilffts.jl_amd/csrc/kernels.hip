@@ -846,6 +846,57 @@ int device_stage_times(Plan& p, int n, float* ms) {
   return PFFT_OK;
 }
 
+// ---- execution without allocation (pfft_execute_noalloc) -------------------------
+// The intermediate pencils of a plan (the workspace of pfft_execute) live in
+// this array instead of caller or plan memory: each GPU the library runs on
+// holds one copy of kNoAllocScratch bytes until the process ends.  Host code
+// never passes `scratch` itself to a kernel or copy -- on the host that name
+// stands for a host-side shadow, not a device address -- but the device
+// address hipGetSymbolAddress gives for the current device, looked up once
+// per device and kept in g_scratch_ptr.  Calls on one GPU share the array:
+// two executes that overlap in time on one device (other streams, other
+// threads) race on it.  Every pass writes its whole output before the next
+// reads it, so nothing in it needs clearing between calls.
+constexpr size_t kNoAllocScratch = size_t(256) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_for_device(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PFFT_ERR_DEVICE, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PFFT_ERR_DEVICE,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PFFT_OK;
+}
+
+int device_execute_noalloc(Plan& p, int dir, const void* in, void* out, hipStream_t stream) {
+  if (p.work_bytes > kNoAllocScratch)
+    return fail(PFFT_ERR_ARGUMENT,
+                "pfft_execute_noalloc: the plan needs " + std::to_string(p.work_bytes) +
+                    " bytes of intermediate storage, above the static scratch of " +
+                    std::to_string(kNoAllocScratch) + " bytes; use pfft_execute");
+  if (!p.dev)
+    return fail(PFFT_ERR_ARGUMENT,
+                "pfft_execute_noalloc: the plan is not prepared (call pfft_plan_prepare "
+                "first: preparing uploads tables and tunes kernels, which allocates)");
+  void* ptr = nullptr;
+  const int r = scratch_for_device(&ptr);
+  if (r) return r;
+  return device_execute(p, dir, in, out, ptr, kNoAllocScratch, stream);
+}
+
 static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
   const char* x = (const char*)a;
   const char* y = (const char*)b;

@@ -117,6 +117,8 @@ int device_prepare(Plan& p);
 void device_release(Plan& p);
 int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size_t work_bytes,
                    hipStream_t stream, const pfft_spectral_op* op = nullptr);
+// pfft_execute with the intermediates in the library's static device scratch
+int device_execute_noalloc(Plan& p, int dir, const void* in, void* out, hipStream_t stream);
 int device_set_timing(Plan& p, bool on);
 int device_project(Plan& p, void* const* uh, const double* scale, hipStream_t stream);
 int device_stage_times(Plan& p, int n, float* ms);
