@@ -140,12 +140,21 @@ def cpu_baseline(dims, transform, real, reps):
         tb.append(time.perf_counter() - t1)
         tf.append(t1 - t0)
     cpu = "unknown"
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                if ":" not in line:
+                    if info:
+                        break          # the first processor's block is enough
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "cpu MHz",
+                         "cache size", "cpu cores", "siblings"):
+                    info[k] = v
+        cpu = "%s (family %s, model %s, stepping %s; %s CPUs visible, %d used)" % (
+            info.get("model name", "unknown"), info.get("cpu family", "?"),
+            info.get("model", "?"), info.get("stepping", "?"), os.cpu_count(), workers)
     except OSError:
         pass
     return {"value": ms, "unit": "ms/pair", "cores": workers, "kind": "port",
@@ -242,24 +251,25 @@ def bandwidth_lib():
     path = os.path.join(ROOT, "pencilffts.jl_amd", "lib", "libpfft_bandwidth.so")
     lib = C.CDLL(path)
     lib.pfft_bandwidth_copy.restype = C.c_int
-    lib.pfft_bandwidth_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
-                                        C.c_void_p]
+    lib.pfft_bandwidth_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int,
+                                        C.c_int, C.c_void_p]
     return lib
 
 
-def stream_copy(torch, lib, dst, src, per_cu):
+def stream_copy(torch, lib, dst, src, per_cu, unroll=8, nt_loads=0):
     st = torch.cuda.current_stream()
     r = lib.pfft_bandwidth_copy(dst.data_ptr(), src.data_ptr(), src.numel() * 4, per_cu,
-                                st.cuda_stream)
+                                unroll, nt_loads, st.cuda_stream)
     if r:
         raise RuntimeError("pfft_bandwidth_copy failed: hipError %d" % r)
 
 
 def copy_ceiling(torch, nbytes=2 << 30, reps=10):
     """Measured HBM ceiling: read + write bytes per second of the
-    hand-written streaming copy (16-byte loads, nontemporal stores,
-    persistent grid) over a 2 GiB buffer, best over resident workgroups per
-    CU; torch's device copy of the same buffer beside it."""
+    hand-written streaming copy (16-byte loads, nontemporal stores) over a
+    2 GiB buffer, the best over its grid shapes (resident workgroups per CU,
+    or one per chunk), vectors in flight per lane and load hints; torch's
+    device copy of the same buffer beside it."""
     lib = bandwidth_lib()
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
@@ -278,17 +288,23 @@ def copy_ceiling(torch, nbytes=2 << 30, reps=10):
         return 2 * nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
 
     sweep = {}
-    for per_cu in (1, 2, 4, 8):
-        sweep[per_cu] = round(timed(lambda: stream_copy(torch, lib, dst, src, per_cu)), 1)
+    for per_cu in (0, 1, 2, 4, 8):
+        for unroll in (4, 8, 16):
+            for ntl in (0, 1):
+                key = "%s,unroll%d%s" % ("grid=chunks" if per_cu == 0 else "%d/CU" % per_cu,
+                                         unroll, ",ntload" if ntl else "")
+                sweep[key] = round(timed(lambda: stream_copy(torch, lib, dst, src, per_cu,
+                                                             unroll, ntl)), 1)
     ok = bool(torch.equal(dst, src))
     torch_gbs = timed(lambda: dst.copy_(src))
     del src, dst
     torch.cuda.empty_cache()
     best = max(sweep, key=sweep.get)
-    return {"GBs": sweep[best], "kernel": "k_stream_copy (16-byte loads, nontemporal stores, "
-                                          "%d workgroups of 256 per CU)" % best,
-            "bytes": 2 * nbytes, "per_cu_sweep_GBs": sweep, "copy_correct": ok,
-            "torch_copy_GBs": round(torch_gbs, 1)}
+    top = dict(sorted(sweep.items(), key=lambda kv: -kv[1])[:6])
+    return {"GBs": sweep[best], "kernel": "k_stream_copy (16-byte loads, nontemporal stores; "
+                                          "%s)" % best,
+            "bytes": 2 * nbytes, "sweep_top_GBs": top, "sweep_min_GBs": min(sweep.values()),
+            "copy_correct": ok, "torch_copy_GBs": round(torch_gbs, 1)}
 
 
 def error_vs_f64(torch, pf, dims, tr, u32):
@@ -365,7 +381,13 @@ def extra_plain(torch, pf, steps, warmup):
         plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, rt)
         u = pf.allocate_input(plan)
         v = pf.allocate_output(plan)
-        fill(torch, u, g)
+        if key.startswith("mixed"):
+            # the mixed workload's data: 32 Fourier-Chebyshev modes + 1e-3 noise
+            u.logical().copy_(pf.workloads.fourier_chebyshev_field(dims, n_modes=32, noise=1e-3,
+                                                                   seed=7))
+            label += ", 32 Fourier-Chebyshev modes + 1e-3 noise"
+        else:
+            fill(torch, u, g)
         res[key] = pair_entry(torch, pf, plan, u, v, steps, warmup, label)
         del plan, u, v
         torch.cuda.empty_cache()
@@ -630,7 +652,7 @@ def main():
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(2):
-            stream_copy(torch, lib, dst, src, 4)
+            stream_copy(torch, lib, dst, src, 0, 8, 0)
         torch.cuda.synchronize()
         del src, dst
     if not args.no_extras:

@@ -247,8 +247,8 @@ int pfft_execute_many(pfft_plan* plan, int32_t direction, int32_t n_arrays,
  * components, as in a divergence or a gradient's adjoint).  No counterpart in
  * the reference's API: it applies these operators in separate Julia loops
  * between transforms.  Plans need FFT/BFFT/RFFT dimensions, and the carrying
- * pass must be a c2c one (two or more transformed dimensions), else
- * PFFT_ERR_UNSUPPORTED.                                                      */
+ * pass must be a c2c one (two or more transformed dimensions) whose lines
+ * fit one LDS-resident pass (not split in two), else PFFT_ERR_UNSUPPORTED.  */
 enum {
   PFFT_OP_NONE = 0,
   PFFT_OP_DERIVATIVE = 1,

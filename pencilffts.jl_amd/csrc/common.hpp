@@ -13,6 +13,10 @@ namespace pfft {
 constexpr int kMaxDims = 8;       // transformed dims + the flattened extra dim
 constexpr int kMaxStages = 24;    // radix stages of one line FFT
 constexpr int kBlockThreads = 256;
+constexpr int64_t kMaxLds = 160 * 1024;  // LDS bytes per workgroup (gfx950)
+// Line FFTs whose length has a prime factor above this run in the Bluestein
+// form (direct stages would cost that factor per output point).
+constexpr int64_t kBluesteinPrime = 64;
 
 // What one pass does to each line.  A pass transforms one logical dimension
 // of the array (the reference's `mul!(parent(v), r.fftw_plan, parent(u))`,
@@ -91,6 +95,20 @@ struct PassArgs {
   int64_t kn[kMaxDims];     // per logical dim: spectrum length
   double kmax[kMaxDims];    // per logical dim: largest stored |k|
   double op_alpha;
+  // Bluestein (chirp-z) form of the M-point line FFT, for lengths with a
+  // large prime factor: X_k = c_k sum_j (x_j c_j) conj(c_{k-j}) with
+  // c_j = exp(-i pi j^2 / M), the convolution done by two bs_M-point FFTs
+  // (bs_M a power of two >= 2M - 1).  bs_M = 0: off.  With it, radix / nstages
+  // / tw describe the bs_M-point FFT.
+  int64_t bs_M;
+  const void* bs_chirp;   // c_j, j < M
+  const void* bs_bhat;    // bs_M-point FFT of b_m = conj(c_m) (m mod bs_M, |m| < M), / bs_M
+  // First pass of the four-step split of a line too long for the LDS: output
+  // k of the line with index j2 along line dim 0 is multiplied by
+  // w^(j2 k), w = exp(-2 pi i / fs_M) (conjugated for a backward
+  // transform).  fs_M = 0: off.
+  int64_t fs_M;
+  const void* fs_tw;      // fs_M-point twiddles
 };
 
 template <typename R>

@@ -30,8 +30,8 @@ static void emu_pass(const PassPlan& pp, const PassArgs& a) {
     tile_init(a, tile, c);
     for (int t = 0; t < nthr; ++t) phase_load<R>(a, c, t, nthr, lds.data());
     for (int t = 0; t < nthr; ++t) phase_pre<R>(a, c, t, nthr, lds.data());
-    for (int s = 0; s < a.nstages; ++s)
-      for (int t = 0; t < nthr; ++t) phase_stage<R>(a, c, s, t, nthr, lds.data());
+    for (int s = 0; s < fft_steps(a); ++s)
+      for (int t = 0; t < nthr; ++t) phase_fft_step<R>(a, c, s, t, nthr, lds.data());
     for (int t = 0; t < nthr; ++t) phase_post<R>(a, c, t, nthr, lds.data());
   }
 }
@@ -97,8 +97,7 @@ __attribute__((visibility("default"))) int pfft_emu_execute_op(pfft_emu_plan* h,
     PassArgs a = pp.args;
     a.in = pp.src == 0 ? in : bufs[pp.src];
     a.out = pp.dst == 0 ? out : bufs[pp.dst];
-    a.tw = pp.tw_id >= 0 ? (const void*)p.tables[pp.tw_id].host.data() : nullptr;
-    a.aux = pp.aux_id >= 0 ? (const void*)p.tables[pp.aux_id].host.data() : nullptr;
+    bind_tables(p, pp, a, true);
     if (dir == PFFT_BACKWARD_SCALED && i == np - 1) {
       a.do_scale = 1;
       a.scale = 1.0 / p.scale;
@@ -120,6 +119,15 @@ __attribute__((visibility("default"))) int pfft_emu_set_traversal(pfft_emu_plan*
   h->tile_order = order ? 1 : 0;
   h->tile_rev = rev ? 1 : 0;
   return PFFT_OK;
+}
+
+// Convolution length of the Bluestein form of each forward pass (0: direct
+// stages); returns the number of passes.
+__attribute__((visibility("default"))) int pfft_emu_bluestein(pfft_emu_plan* h, int64_t* out,
+                                                             int32_t n) {
+  const int np = (int)h->p.fwd.size();
+  for (int i = 0; i < np && i < n; ++i) out[i] = h->p.fwd[i].args.bs_M;
+  return np;
 }
 
 __attribute__((visibility("default"))) int pfft_emu_execute(pfft_emu_plan* h, int32_t dir,

@@ -544,26 +544,35 @@ __device__ __forceinline__ f16x8 neg8(f16x8 v) {
   return __builtin_bit_cast(f16x8, __builtin_bit_cast(s16x8, v) ^ (short)0x8000);
 }
 
-struct Mfma32Const {   // this lane's slice of the 32-point DFT matrix, both steps
-  f16x8 rh[2], rl[2], ih[2], il[2];
-  __device__ void init(int l) {
+// The DFT-matrix slices of the two steps live in LDS, 8 KB per workgroup,
+// rather than in 32 registers of every lane: entry 64 s + l is lane l's
+// slice s = 2 part + c of part (Re hi, Re lo, Im hi, Im lo) and K-chunk c,
+// i.e. w32^((l mod 32) pi(8 (l div 32) + t + 16 c)) for t = 0..7.  Each
+// operand is read (16 bytes per lane, conflict-free) just before its MFMAs.
+constexpr int kMfma32TableBytes = 8 * 64 * 16;
+__device__ __forceinline__ void mfma32_table_init(f16x8* tab, int tid, int nthr) {
+  for (int idx = tid; idx < 128; idx += nthr) {
+    const int l = idx & 63, c = idx >> 6;
     const int j = l & 31, h = l >> 5;
+    f16x8 rh, rl, ih, il;
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int n = 4 * h + (t & 3) + 8 * (t >> 2) + 16 * c;
-        const int m = (j * n) & 31;
-        const float fr = kCos32[m];
-        const float fi = -kCos32[(m + 24) & 31];   // -sin(2 pi m / 32)
-        const _Float16 a = (_Float16)fr, b = (_Float16)fi;
-        rh[c][t] = a;
-        rl[c][t] = (_Float16)(fr - (float)a);
-        ih[c][t] = b;
-        il[c][t] = (_Float16)(fi - (float)b);
-      }
+    for (int t = 0; t < 8; ++t) {
+      const int n = 4 * h + (t & 3) + 8 * (t >> 2) + 16 * c;
+      const int m = (j * n) & 31;
+      const float fr = kCos32[m];
+      const float fi = -kCos32[(m + 24) & 31];   // -sin(2 pi m / 32)
+      const _Float16 a = (_Float16)fr, b = (_Float16)fi;
+      rh[t] = a;
+      rl[t] = (_Float16)(fr - (float)a);
+      ih[t] = b;
+      il[t] = (_Float16)(fi - (float)b);
+    }
+    tab[(0 + c) * 64 + l] = rh;
+    tab[(2 + c) * 64 + l] = rl;
+    tab[(4 + c) * 64 + l] = ih;
+    tab[(6 + c) * 64 + l] = il;
   }
-};
+}
 
 // D += A B with A, B split in hi + lo
 __device__ __forceinline__ f32x16 mm3(f16x8 ah, f16x8 al, f16x8 bh, f16x8 bl, f32x16 d) {
@@ -575,9 +584,11 @@ __device__ __forceinline__ f32x16 mm3(f16x8 ah, f16x8 al, f16x8 bh, f16x8 bl, f3
 
 // v: point J(l, p) in register p on entry, X[J(l, p)] on exit (times 2^e).
 // tw2: the step twiddles, entry 64 i + l = w1024^((l & 31) n1), n1 the row
-// of accumulator register i.  Returns e.
+// of accumulator register i; ftab: the DFT-matrix slices (LDS).  Returns e.
+// Chunk by chunk, each operand is built (and each slice read) right before
+// the MFMAs that use it, so that few values are live at once.
 __device__ __forceinline__ int fft1024_mfma32(C<float>* v, int l, const C<float>* __restrict__ tw2,
-                                              const Mfma32Const& F) {
+                                              const f16x8* ftab) {
   // scale: largest magnitude as bits (|x| orders like its bit pattern)
   unsigned mb = 0;
 #pragma unroll
@@ -587,6 +598,7 @@ __device__ __forceinline__ int fft1024_mfma32(C<float>* v, int l, const C<float>
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o, 64));
   const int e = mb ? 8 - ((int)(mb >> 23) - 127) : 0;
+  auto F = [&](int part, int c) { return ftab[(2 * part + c) * 64 + l]; };
   f32x16 dr = {}, di = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -600,36 +612,38 @@ __device__ __forceinline__ int fft1024_mfma32(C<float>* v, int l, const C<float>
     split8(xr, rh, rl);
     split8(xi, ih, il);
     // Re Y = Xr Fr - Xi Fi, Im Y = Xr Fi + Xi Fr (data as the A operand)
-    dr = mm3(rh, rl, F.rh[c], F.rl[c], dr);
-    dr = mm3(neg8(ih), neg8(il), F.ih[c], F.il[c], dr);
-    di = mm3(rh, rl, F.ih[c], F.il[c], di);
-    di = mm3(ih, il, F.rh[c], F.rl[c], di);
-  }
-  // twiddle w1024^(n1 k2)
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const C<float> w = tw2[64 * i + l];
-    const float a = dr[i], b = di[i];
-    dr[i] = a * w.re - b * w.im;
-    di[i] = a * w.im + b * w.re;
+    const f16x8 frh = F(0, c), frl = F(1, c);
+    dr = mm3(rh, rl, frh, frl, dr);
+    di = mm3(ih, il, frh, frl, di);
+    const f16x8 fih = F(2, c), fil = F(3, c);
+    dr = mm3(neg8(ih), neg8(il), fih, fil, dr);
+    di = mm3(rh, rl, fih, fil, di);
+    __builtin_amdgcn_sched_barrier(0);
   }
   f32x16 zr = {}, zi = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
+    // twiddle w1024^(n1 k2) on this chunk's rows, then the split
     float yr[8], yi[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      yr[t] = dr[8 * c + t];
-      yi[t] = di[8 * c + t];
+      const int i = 8 * c + t;
+      const C<float> w = tw2[64 * i + l];
+      const float a = dr[i], b = di[i];
+      yr[t] = a * w.re - b * w.im;
+      yi[t] = a * w.im + b * w.re;
     }
     f16x8 rh, rl, ih, il;
     split8(yr, rh, rl);
     split8(yi, ih, il);
     // the DFT matrix as the A operand now
-    zr = mm3(F.rh[c], F.rl[c], rh, rl, zr);
-    zr = mm3(F.ih[c], F.il[c], neg8(ih), neg8(il), zr);
-    zi = mm3(F.rh[c], F.rl[c], ih, il, zi);
-    zi = mm3(F.ih[c], F.il[c], rh, rl, zi);
+    const f16x8 frh = F(0, c), frl = F(1, c);
+    zr = mm3(frh, frl, rh, rl, zr);
+    zi = mm3(frh, frl, ih, il, zi);
+    const f16x8 fih = F(2, c), fil = F(3, c);
+    zr = mm3(fih, fil, neg8(ih), neg8(il), zr);
+    zi = mm3(fih, fil, rh, rl, zi);
+    __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int p = 0; p < 16; ++p) v[p] = mk(zr[p], zi[p]);
@@ -844,20 +858,25 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 // tile needs 1 / NSP of the rows in LDS and more workgroups fit on a CU.
 // PF: the prefetching variants keep the twiddles (and the c2r merge factors)
 // in LDS after the tile / line slots.  The 32 x 32 MFMA lines (MF = 3) keep
-// the whole line in registers and use no line slot.
-template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int PF = 0,
-          int MF = 0>
-constexpr size_t lds_bytes() {   // T / LPW line slots whatever the lanes per line
+// the whole line in registers and use no line slot; their DFT-matrix table
+// follows the tile.
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int MF = 0>
+constexpr size_t lds_base() {   // T / LPW line slots whatever the lanes per line
   constexpr size_t lines = MF == 3 ? 0 : (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
   constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
   constexpr int rows_st = M / NSP + (KIND == K_R2C ? 1 : 0);
   constexpr size_t tld = LD == IO_TILE ? (size_t)rows_ld * T * sizeof(C<R>) : 0;
   constexpr size_t tst = ST == IO_TILE ? (size_t)rows_st * T * sizeof(C<R>) : 0;
   constexpr size_t tile = tld > tst ? tld : tst;
-  constexpr size_t base = lines > tile ? lines : tile;
+  return lines > tile ? lines : tile;
+}
+template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int PF = 0,
+          int MF = 0>
+constexpr size_t lds_bytes() {
   constexpr size_t tables =
       PF ? (size_t)(M + (KIND == K_C2R || KIND == K_R2C ? M + 1 : 0)) * sizeof(C<R>) : 0;
-  return base + tables;
+  constexpr size_t mf3 = MF == 3 ? kMfma32TableBytes : 0;
+  return lds_base<R, M, T, LPW, KIND, LD, ST, NSP, MF>() + tables + mf3;
 }
 
 // Register staging of a transposed input tile (prefetching variants): the
@@ -978,7 +997,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
                                            : KIND == K_C2C || KIND == K_R2C)),
                 "prefetching variants: c2c, c2r with transposed loads, r2c with line loads");
   constexpr bool PFT = PF && LD == IO_TILE, PFL = PF && LD == IO_LINE;
-  constexpr size_t TBL = lds_bytes<R, M, T, LPW, KIND, LD, ST, NSP, 0, MF>();
+  constexpr size_t TBL = lds_base<R, M, T, LPW, KIND, LD, ST, NSP, MF>();
   if constexpr (PF) {
     C<R>* st = (C<R>*)(smem + TBL);
     for (int i = tid; i < M; i += NTHR) st[i] = ((const C<R>*)a.tw)[i];
@@ -996,8 +1015,12 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   static_assert(MF == 0 || (M == 1024 && sizeof(R) == 4), "MFMA stages: 1024-point f32 lines");
   MfmaA<MF == 1 || MF == 2 ? MF : 1> mfA;
   if constexpr (MF == 1 || MF == 2) mfA.init(l % 64);
-  [[maybe_unused]] Mfma32Const mf32;
-  if constexpr (JM) mf32.init(l);
+  // the 32 x 32 MFMA form's DFT-matrix slices, after the tile
+  [[maybe_unused]] f16x8* const ftab = (f16x8*)(smem + TBL);
+  if constexpr (JM) {
+    mfma32_table_init(ftab, tid, NTHR);
+    __syncthreads();
+  }
 
   [[maybe_unused]] TileStage<R, T, PFT ? ROWS_IN : 1, NTHR, SW> stg;
   // the next tile's lines (contiguous loads), as native vectors
@@ -1052,6 +1075,9 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     }
 #pragma unroll
     for (int q = 0; q < LPW; ++q) {
+      // the 32 x 32 MFMA lines load each line right before its transform
+      // (below), so that one line's registers are live at a time
+      if constexpr (JM) break;
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
       if constexpr (GATHER) {
@@ -1177,10 +1203,29 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       const bool active = t < c.Tact;
       if (q > 0) line_sync<L>();
       [[maybe_unused]] R scq = sc;   // output scaling of this line
+      if constexpr (JM) {
+        // this line's points J(l, p), from the tile or from global memory
+        if constexpr (LD == IO_TILE) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const C<R> x = TK(p, t, 0);
+            v[q][p] = conj_in ? conj(x) : x;
+          }
+        } else {
+          const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+          if (active) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              const C<R> x = in[lrow + ROW(p)];
+              v[q][p] = conj_in ? conj(x) : x;
+            }
+          }
+        }
+      }
       if constexpr (MF != 0) {
         int e;
         if constexpr (JM)
-          e = fft1024_mfma32((C<float>*)v[q], l, (const C<float>*)aux, mf32);
+          e = fft1024_mfma32((C<float>*)v[q], l, (const C<float>*)aux, ftab);
         else
           e = fft1024_mfma<MF, L>((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw, mfA);
         // remove the entry scaling 2^e: folded into the output factor when
@@ -1298,6 +1343,11 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
               for (int p = 0; p < P; ++p) st_line<NT != 0>(&out[lrow + ROW(p)], v[q][p]);
             }
           }
+        } else if constexpr (JM && NSP == 1) {
+          // the whole tile is in LDS: the line goes there now and its
+          // registers are free for the next line
+#pragma unroll
+          for (int p = 0; p < P; ++p) TK(p, t, 0) = v[q][p];
         }
       }
     }
@@ -1312,6 +1362,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         __syncthreads();   // line slots / the previous chunk are read
 #pragma unroll
         for (int q = 0; q < LPW; ++q) {
+          if constexpr (JM && NSP == 1) break;   // written line by line above
           const int t = q * TQ + g;
 #pragma unroll
           for (int p = h * PCH; p < (h + 1) * PCH; ++p) TK(p, t, -h * RCH) = v[q][p];
@@ -1397,6 +1448,18 @@ static void add_mfma32(std::vector<Entry>& v) {
   add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
   add<float, 1024, 2, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
   add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_LINE, 3>(v);
+  // register budgets for 4 and 3 waves per SIMD
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, 3, 1, 1, 4>(v);
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, 3, 1, 1, 3>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 3, 2, 1, 4>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 3, 2, 1, 3>(v);
+  // 16-line tiles (128-byte runs) with several lines per wavefront, one line
+  // in registers at a time: each goes into the whole-tile LDS buffer as soon
+  // as it is transformed
+  add<float, 1024, 16, 2, K_C2C, IO_LINE, IO_TILE, 3>(v);
+  add<float, 1024, 16, 4, K_C2C, IO_LINE, IO_TILE, 3>(v);
+  add<float, 1024, 8, 2, K_C2C, IO_LINE, IO_TILE, 3>(v);
+  add<float, 1024, 16, 2, K_C2C, IO_TILE, IO_LINE, 3>(v);
   add<float, 1024, 16, 1, K_C2C, IO_LINE, IO_TILE, 3, 2>(v);
   add<float, 1024, 16, 1, K_C2C, IO_LINE, IO_TILE, 3, 4>(v);
   add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 3>(v);
@@ -1492,6 +1555,14 @@ static void add_size(std::vector<Entry>& v) {
       if constexpr (M <= 256) {   // larger lines spill at 128 VGPRs
         add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE>(v);
         add<R, M, 32, 2, K_C2R, IO_TILE, IO_LINE>(v);
+      }
+      if constexpr (M == 256 || M == 512) {
+        // 256-byte runs (32-line tiles) in row chunks, so the tile takes a
+        // quarter (a half) of the LDS of the whole-tile form
+        add<R, M, 32, 4, K_C2C, IO_LINE, IO_TILE, 0, 4>(v);
+        add<R, M, 32, 4, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+        add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE, 0, 2>(v);
+        add<R, M, 32, 2, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
       }
     }
   } else {
@@ -1654,6 +1725,8 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
   std::vector<int> out;
   if (p.flags & PFFT_FLAG_GENERIC_ONLY) return out;
   PassArgs a = pp.args;
+  // the Bluestein form and the four-step twiddle run on the generic pass only
+  if (a.bs_M || a.fs_M) return out;
   int kind;
   switch (a.kind) {
     case L_C2C: kind = K_C2C; break;

@@ -21,6 +21,7 @@ struct TileCtx {
   int64_t in_base;   // element offset of the tile's first line (input)
   int64_t out_base;  // element offset of the tile's first line (output)
   int64_t tile;      // tile index
+  int64_t m0;        // index along line dim 0 of the tile's first line
   int32_t Tact;      // lines present in this tile
   int32_t Tw;        // work lines (two per line for complex R2R data)
 };
@@ -133,6 +134,7 @@ PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
   const int64_t left = a.ldim[0] - b0 * a.T;
   c.Tact = (int32_t)(left < a.T ? left : a.T);
   c.tile = tile;
+  c.m0 = b0 * a.T;
   c.Tw = a.r2r_cplx ? 2 * c.Tact : c.Tact;
 }
 
@@ -325,14 +327,13 @@ PFFT_HD void dft8(cpx<R>* v) {
 }
 
 template <typename R, int RAD>
-PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns,
+PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_t fftM, bool bwd,
                          const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
   const cpx<R>* tw = (const cpx<R>*)a.tw;
-  const uint32_t M = (uint32_t)a.M;
+  const uint32_t M = (uint32_t)fftM;
   const uint32_t nb = M / RAD;                 // butterflies per line
   const uint32_t step = (uint32_t)(M / (Ns * RAD));
   const uint32_t count = (uint32_t)c.Tw * nb;
-  const bool bwd = a.bwd != 0;
   for (uint32_t e = tid; e < count; e += nthr) {
     const uint32_t w = e / nb;
     const uint32_t j = e - w * nb;
@@ -356,14 +357,13 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns,
 
 // Any radix (e.g. a prime factor 7, 11, 41): one work item per output.
 template <typename R>
-PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
-                       const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
+PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD, int64_t fftM,
+                       bool bwd, const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
   const cpx<R>* tw = (const cpx<R>*)a.tw;
-  const uint32_t M = (uint32_t)a.M;
+  const uint32_t M = (uint32_t)fftM;
   const uint32_t nb = M / RAD;
   const uint32_t step = (uint32_t)(M / (Ns * RAD));
   const uint32_t count = (uint32_t)c.Tw * M;
-  const bool bwd = a.bwd != 0;
   for (uint32_t e = tid; e < count; e += nthr) {
     const uint32_t w = e / M;
     const uint32_t rem = e - w * M;
@@ -384,21 +384,105 @@ PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
   }
 }
 
+// Stage s of the line FFT (fftM points, radix schedule a.radix), from the
+// work buffer of parity sb to the other one; bwd: backward butterflies.
+template <typename R>
+PFFT_HD void phase_stage_ex(const PassArgs& a, const TileCtx& c, int s, int sb, int64_t fftM,
+                            bool bwd, int tid, int nthr, char* lds) {
+  int64_t Ns = 1;
+  for (int i = 0; i < s; ++i) Ns *= a.radix[i];
+  const cpx<R>* src = (const cpx<R>*)(lds + ((sb & 1) ? a.lds_w1_off : 0));
+  cpx<R>* dst = (cpx<R>*)(lds + ((sb & 1) ? 0 : a.lds_w1_off));
+  switch (a.radix[s]) {
+    case 2: stage_fixed<R, 2>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
+    case 3: stage_fixed<R, 3>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
+    case 4: stage_fixed<R, 4>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
+    case 5: stage_fixed<R, 5>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
+    case 8: stage_fixed<R, 8>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
+    default: stage_any<R>(a, c, Ns, a.radix[s], fftM, bwd, src, dst, tid, nthr); break;
+  }
+}
+
+// Stage s of the direct M-point line FFT.
 template <typename R>
 PFFT_HD void phase_stage(const PassArgs& a, const TileCtx& c, int s, int tid,
                          int nthr, char* lds) {
-  int64_t Ns = 1;
-  for (int i = 0; i < s; ++i) Ns *= a.radix[i];
-  const cpx<R>* src = (const cpx<R>*)(lds + ((s & 1) ? a.lds_w1_off : 0));
-  cpx<R>* dst = (cpx<R>*)(lds + ((s & 1) ? 0 : a.lds_w1_off));
-  switch (a.radix[s]) {
-    case 2: stage_fixed<R, 2>(a, c, Ns, src, dst, tid, nthr); break;
-    case 3: stage_fixed<R, 3>(a, c, Ns, src, dst, tid, nthr); break;
-    case 4: stage_fixed<R, 4>(a, c, Ns, src, dst, tid, nthr); break;
-    case 5: stage_fixed<R, 5>(a, c, Ns, src, dst, tid, nthr); break;
-    case 8: stage_fixed<R, 8>(a, c, Ns, src, dst, tid, nthr); break;
-    default: stage_any<R>(a, c, Ns, a.radix[s], src, dst, tid, nthr); break;
+  phase_stage_ex<R>(a, c, s, s, a.M, a.bwd != 0, tid, nthr, lds);
+}
+
+// ---- Bluestein form of the line FFT ----------------------------------------
+// The M-point transform of the pre-processed lines in work buffer 0, as
+//   bs_pre   a_j = x_j c_j (x conjugated first for a backward transform),
+//            zero-padded to bs_M points, in place;
+//   stages   the bs_M-point forward FFT;
+//   bs_mul   times the transform of b (its 1 / bs_M included);
+//   stages   the bs_M-point backward FFT (the stage parity continues, so
+//            the result is back in work buffer 0 after the two FFTs);
+//   bs_post  X_k = c_k C_k (conjugated back for a backward transform), k < M.
+template <typename R>
+PFFT_HD void phase_bs_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr, char* lds) {
+  cpx<R>* W = (cpx<R>*)lds;
+  const cpx<R>* ch = (const cpx<R>*)a.bs_chirp;
+  const uint32_t Mb = (uint32_t)a.bs_M, M = (uint32_t)a.M;
+  const uint32_t count = (uint32_t)c.Tw * Mb;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / Mb, j = e - w * Mb;
+    cpx<R>* x = W + w * a.Mp + j;
+    if (j < M) {
+      const cpx<R> v = a.bwd ? conj(*x) : *x;
+      *x = v * ch[j];
+    } else {
+      *x = mk(R(0), R(0));
+    }
   }
+}
+
+template <typename R>
+PFFT_HD void phase_bs_mul(const PassArgs& a, const TileCtx& c, int tid, int nthr, char* lds) {
+  const int ns = a.nstages;
+  cpx<R>* W = (cpx<R>*)(lds + ((ns & 1) ? a.lds_w1_off : 0));
+  const cpx<R>* bh = (const cpx<R>*)a.bs_bhat;
+  const uint32_t Mb = (uint32_t)a.bs_M;
+  const uint32_t count = (uint32_t)c.Tw * Mb;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / Mb, j = e - w * Mb;
+    cpx<R>* x = W + w * a.Mp + j;
+    *x = *x * bh[j];
+  }
+}
+
+template <typename R>
+PFFT_HD void phase_bs_post(const PassArgs& a, const TileCtx& c, int tid, int nthr, char* lds) {
+  cpx<R>* W = (cpx<R>*)lds;
+  const cpx<R>* ch = (const cpx<R>*)a.bs_chirp;
+  const uint32_t M = (uint32_t)a.M;
+  const uint32_t count = (uint32_t)c.Tw * M;
+  for (uint32_t e = tid; e < count; e += nthr) {
+    const uint32_t w = e / M, k = e - w * M;
+    cpx<R>* x = W + w * a.Mp + k;
+    const cpx<R> v = *x * ch[k];
+    *x = a.bwd ? conj(v) : v;
+  }
+}
+
+// ---- phase 3: the line FFT, as a sequence of barrier-separated steps -------
+// Direct form: the radix stages.  Bluestein form: bs_pre, the forward
+// stages, bs_mul, the backward stages, bs_post.
+PFFT_HD int fft_steps(const PassArgs& a) {
+  return a.bs_M ? 2 * a.nstages + 3 : a.nstages;
+}
+
+template <typename R>
+PFFT_HD void phase_fft_step(const PassArgs& a, const TileCtx& c, int i, int tid, int nthr,
+                            char* lds) {
+  if (!a.bs_M) { phase_stage<R>(a, c, i, tid, nthr, lds); return; }
+  const int ns = a.nstages;
+  if (i == 0) phase_bs_pre<R>(a, c, tid, nthr, lds);
+  else if (i <= ns) phase_stage_ex<R>(a, c, i - 1, i - 1, a.bs_M, false, tid, nthr, lds);
+  else if (i == ns + 1) phase_bs_mul<R>(a, c, tid, nthr, lds);
+  else if (i <= 2 * ns + 1)
+    phase_stage_ex<R>(a, c, i - ns - 2, i - 2, a.bs_M, true, tid, nthr, lds);
+  else phase_bs_post<R>(a, c, tid, nthr, lds);
 }
 
 // ---- phase 4: post-process and store --------------------------------------
@@ -466,7 +550,10 @@ PFFT_HD cpx<R> post_cplx(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t
 template <typename R>
 PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
                         char* lds) {
-  const cpx<R>* W = (const cpx<R>*)(lds + ((a.nstages & 1) ? a.lds_w1_off : 0));
+  // the line FFT's output: the buffer its last stage wrote (work buffer 0
+  // after the Bluestein form's two FFTs)
+  const cpx<R>* W =
+      (const cpx<R>*)(lds + ((a.bs_M == 0 && (a.nstages & 1)) ? a.lds_w1_off : 0));
   const R* raw = (const R*)(lds + a.lds_w1_off);
   const uint32_t L = (uint32_t)a.L_out;
   const uint32_t Tact = (uint32_t)c.Tact;
@@ -483,6 +570,10 @@ PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       if (a.kind == L_COPY) y = raw_cplx(a, raw, t, k);
       else if (a.r2r_cplx) y = mk(post_real(a, W, 2 * t, k), post_real(a, W, 2 * t + 1, k));
       else y = post_cplx(a, W, t, k);
+      if (a.fs_M) {
+        const cpx<R> w = ((const cpx<R>*)a.fs_tw)[(c.m0 + t) * k];
+        y = y * (a.bwd ? conj(w) : w);
+      }
       if (a.do_scale) y = scl(y, sc);
       if (a.op && a.op_side == 2) {
         int64_t m0;

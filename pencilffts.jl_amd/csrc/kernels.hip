@@ -30,8 +30,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_generic(PassArgs a) {
     __syncthreads();
     phase_pre<R>(a, c, threadIdx.x, blockDim.x, lds);
     __syncthreads();
-    for (int s = 0; s < a.nstages; ++s) {
-      phase_stage<R>(a, c, s, threadIdx.x, blockDim.x, lds);
+    const int nsteps = fft_steps(a);
+    for (int s = 0; s < nsteps; ++s) {
+      phase_fft_step<R>(a, c, s, threadIdx.x, blockDim.x, lds);
       __syncthreads();
     }
     phase_post<R>(a, c, threadIdx.x, blockDim.x, lds);
@@ -119,15 +120,13 @@ static int time_variant(Plan& p, const PassPlan& base, int variant, void* in, vo
   }
   a.in = in;
   a.out = out;
-  a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
-  a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+  bind_tables(p, pp, a);
   PassArgs pa;
   if (prod) {
     pa = prod->args;
     pa.in = prod_in;
     pa.out = in;
-    pa.tw = prod->tw_id >= 0 ? p.tables[prod->tw_id].dev : nullptr;
-    pa.aux = prod->aux_id >= 0 ? p.tables[prod->aux_id].dev : nullptr;
+    bind_tables(p, *prod, pa);
   }
   auto produce = [&]() { return prod ? fast_launch(*prod, pa, st) : PFFT_OK; };
   r = produce();
@@ -176,8 +175,7 @@ static int launch_pass(const Plan& p, const PassPlan& pp, const void* src, void*
   PassArgs a = pp.args;
   a.in = src;
   a.out = dst;
-  a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
-  a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+  bind_tables(p, pp, a);
   if (scale) {
     a.do_scale = 1;
     a.scale = 1.0 / p.scale;
@@ -507,13 +505,17 @@ static int autotune(Plan& p) {
       }
       std::vector<PassPlan>& list = dir == PFFT_FORWARD ? p.fwd : p.bwd;
       PassPlan& pp = list[idx];
-      if (pp.variant == 0) continue;   // the generic pass carries it
       PassPlan q = pp;
       q.args = q.gen_args;
       q.threads = q.gen_threads;
       q.lds_bytes = q.gen_lds;
       q.grid = q.gen_grid;
       q.variant = 0;
+      // a pass without fast variants runs (and carries the operator) on the
+      // generic kernel; the others are considered whether or not the plain
+      // pass was settled from the cache above, so that a fresh and a warm
+      // tune cache tune the same set
+      if (fast_candidates(p, q, false).empty()) continue;
       consider(pp, q, side, idx > 0 ? &list[idx - 1] : nullptr, dir == PFFT_FORWARD ? 0 : 1,
                idx);
     }
@@ -722,6 +724,8 @@ int device_pass_names(Plan& p, int dir, std::string* out) {
     *out += passes[i].variant == 0
                 ? std::string(p.real_type == PFFT_F32 ? "k_generic<f32>" : "k_generic<f64>")
                 : std::string(fast_name(passes[i].variant));
+    if (passes[i].variant == 0 && passes[i].args.bs_M)
+      *out += "/bluestein" + std::to_string(passes[i].args.bs_M);
   }
   return PFFT_OK;
 }
@@ -983,8 +987,7 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
     PassArgs a = pp.args;
     a.in = pp.src == 0 ? in : bufs[pp.src];
     a.out = pp.dst == 0 ? out : bufs[pp.dst];
-    a.tw = pp.tw_id >= 0 ? p.tables[pp.tw_id].dev : nullptr;
-    a.aux = pp.aux_id >= 0 ? p.tables[pp.aux_id].dev : nullptr;
+    bind_tables(p, pp, a);
     if (dir == PFFT_BACKWARD_SCALED && i == np - 1) {
       a.do_scale = 1;
       a.scale = 1.0 / p.scale;

@@ -270,6 +270,57 @@ static int twiddle_table(Plan& p, int64_t M) {
   return add_table(p, "tw", std::move(v));
 }
 
+// Tables of the Bluestein form of an M-point FFT convolved at Mb points:
+// the chirp c_j = exp(-i pi j^2 / M), j < M (argument j^2 mod 2M reduced
+// exactly), and the Mb-point DFT of b_m = conj(c_|m|) (|m| < M, index m mod
+// Mb, zero elsewhere) divided by Mb, from a radix-2 FFT in long double.
+static void bluestein_tables(Plan& p, int64_t M, int64_t Mb, int32_t* chirp_id,
+                             int32_t* bhat_id) {
+  std::vector<double> ch(2 * M);
+  for (int64_t j = 0; j < M; ++j) expm((j * j) % (2 * M), 2 * M, &ch[2 * j], &ch[2 * j + 1]);
+  std::vector<long double> re(Mb, 0.0L), im(Mb, 0.0L);
+  for (int64_t m = 0; m < M; ++m) {
+    re[m] = ch[2 * m];
+    im[m] = -(long double)ch[2 * m + 1];
+    if (m) { re[Mb - m] = re[m]; im[Mb - m] = im[m]; }
+  }
+  // in-place decimation-in-time FFT, bit-reversed input order
+  for (int64_t i = 1, j = 0; i < Mb; ++i) {
+    int64_t bit = Mb >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+  }
+  const long double tau = 6.283185307179586476925286766559005768L;
+  for (int64_t len = 2; len <= Mb; len <<= 1) {
+    const int64_t h = len / 2;
+    for (int64_t k = 0; k < h; ++k) {
+      const long double wr = cosl(tau * (long double)k / (long double)len);
+      const long double wi = -sinl(tau * (long double)k / (long double)len);
+      for (int64_t s = 0; s < Mb; s += len) {
+        const int64_t u = s + k, v = s + k + h;
+        const long double tr = re[v] * wr - im[v] * wi, ti = re[v] * wi + im[v] * wr;
+        re[v] = re[u] - tr; im[v] = im[u] - ti;
+        re[u] += tr; im[u] += ti;
+      }
+    }
+  }
+  std::vector<double> bh(2 * Mb);
+  for (int64_t k = 0; k < Mb; ++k) {
+    bh[2 * k] = (double)(re[k] / (long double)Mb);
+    bh[2 * k + 1] = (double)(im[k] / (long double)Mb);
+  }
+  *chirp_id = add_table(p, "bs_chirp", std::move(ch));
+  *bhat_id = add_table(p, "bs_bhat", std::move(bh));
+}
+
+static int64_t largest_prime_factor(int64_t m) {
+  int64_t best = 1;
+  for (int64_t q = 2; q * q <= m; ++q)
+    while (m % q == 0) { best = q; m /= q; }
+  return m > 1 ? std::max(best, m) : best;
+}
+
 // ---- radix schedule ----------------------------------------------------------
 static int radix_schedule(int64_t M, int32_t* rad) {
   int ns = 0;
@@ -327,11 +378,236 @@ static void chain_perm(int N, int n, bool permute, int32_t* perm) {
 
 // Build the pass for transforming logical dim `dim` from (perm_in, dims_in)
 // into (perm_out, dims_out).
+// Radix schedule, twiddles and per-kind tables of a pass's M-point line
+// FFT (a.kind, a.n, a.M, a.raw_pitch set).
+static int line_fft(Plan& p, PassPlan& pp) {
+  PassArgs& a = pp.args;
+  const int lk = a.kind;
+  const int64_t M = a.M, n = a.n;
+  const int64_t rs = p.real_type == PFFT_F32 ? 4 : 8;
+  const int64_t cs = 2 * rs;
+  // A large prime factor makes the direct stages cost M p per line; the
+  // Bluestein form does the line in two FFTs of the next power of two at
+  // or above 2M - 1 instead, when one line of it fits the LDS.
+  int64_t Mb = 0;
+  if (largest_prime_factor(M) > kBluesteinPrime) {
+    Mb = 1;
+    while (Mb < 2 * M - 1) Mb *= 2;
+    const int64_t Tw1 = a.r2r_cplx ? 2 : 1;
+    const int64_t w0 = (Tw1 * (Mb + 1) * cs + 255) / 256 * 256;
+    if (w0 + std::max(Tw1 * (Mb + 1) * cs, a.raw_pitch * rs) > kMaxLds) Mb = 0;
+  }
+  const int ns = radix_schedule(Mb ? Mb : M, a.radix);
+  if (ns < 0) return fail(PFFT_ERR_UNSUPPORTED, "too many radix stages");
+  a.nstages = ns;
+  pp.tw_id = twiddle_table(p, Mb ? Mb : M);
+  if (Mb) {
+    a.bs_M = Mb;
+    bluestein_tables(p, M, Mb, &pp.bs_chirp_id, &pp.bs_bhat_id);
+  }
+  // aux tables
+  std::vector<double> aux;
+  if (lk == L_R2C_HALF || lk == L_C2R_HALF) {
+    aux.resize(2 * (M + 1));
+    for (int64_t k = 0; k <= M; ++k) expm(k, n, &aux[2 * k], &aux[2 * k + 1]);
+  } else if (lk == L_DCT2 || lk == L_DST2 || lk == L_DCT3 || lk == L_DST3) {
+    aux.resize(2 * n);
+    for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
+  } else if (lk == L_DCT4 || lk == L_DST4) {
+    aux.resize(4 * n);
+    for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
+    for (int64_t k = 0; k < n; ++k)
+      expm(2 * k + 1, 8 * n, &aux[2 * (n + k)], &aux[2 * (n + k) + 1]);
+  } else if (lk == L_C2C && M == 1024 && p.real_type == PFFT_F32) {
+    // middle twiddles of the 32 x 32 MFMA form of 1024-point lines, in the
+    // order its lanes read them: entry 64 i + l is w1024^(k2 n1) with
+    // k2 = l mod 32 and n1 = (i mod 4) + 8 (i div 4) + 4 (l div 32), the
+    // row of accumulator register i of lane l
+    aux.resize(2 * 1024);
+    for (int i = 0; i < 16; ++i)
+      for (int l = 0; l < 64; ++l) {
+        const int64_t k2 = l & 31, n1 = (i & 3) + 8 * (i >> 2) + 4 * (l >> 5);
+        expm(k2 * n1, 1024, &aux[2 * (64 * i + l)], &aux[2 * (64 * i + l) + 1]);
+      }
+  }
+  if (!aux.empty()) pp.aux_id = add_table(p, "aux", std::move(aux));
+  return PFFT_OK;
+}
+
+// Line dims of a pass: length and input / output strides (elements), and the
+// logical dim (N: the flattened extra dims, -1: none).
+struct LineDim {
+  int64_t len, si, so;
+  int logical;
+};
+
+// Line-dim order, tile size and LDS footprint of the generic pass (`dim`:
+// the logical dim transformed).  *fits = false when one line does not fit
+// the LDS.
+static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim, bool* fits) {
+  PassArgs& a = pp.args;
+  const int N = p.N;
+  const int lk = a.kind;
+  const int64_t M = a.M;
+  const int64_t rs = p.real_type == PFFT_F32 ? 4 : 8;
+  const int64_t cs = 2 * rs;
+  using LD = LineDim;
+  *fits = true;
+  // Line dims in input-memory order; then the tile axis goes first: the
+  // output's contiguous axis when the store transposes, else the input's
+  // contiguous axis when the load is strided, else the line dim nearest in
+  // input memory (consecutive lines are then adjacent in memory).
+  std::stable_sort(lds.begin(), lds.end(), [](const LD& x, const LD& y) { return x.si < y.si; });
+  int tile = -1;
+  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
+    if (a.sk_out != 1 && lds[i].so == 1 && lds[i].len > 1) tile = (int)i;
+  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
+    if (a.sk_in != 1 && lds[i].si == 1 && lds[i].len > 1) tile = (int)i;
+  if (tile > 0) {
+    const LD t = lds[tile];
+    lds.erase(lds.begin() + tile);
+    lds.insert(lds.begin(), t);
+  }
+  // drop unit dims except the first
+  std::vector<LD> keep;
+  for (size_t i = 0; i < lds.size(); ++i)
+    if (i == 0 || lds[i].len > 1) keep.push_back(lds[i]);
+  if ((int)keep.size() > kMaxDims) return fail(PFFT_ERR_UNSUPPORTED, "too many dimensions");
+  a.D = (int32_t)keep.size();
+  for (int i = 0; i < a.D; ++i) {
+    a.ldim[i] = keep[i].len;
+    a.lsin[i] = keep[i].si;
+    a.lsout[i] = keep[i].so;
+    a.llog[i] = (keep[i].logical >= 0 && keep[i].logical < N) ? keep[i].logical : -1;
+  }
+  a.pdim = dim;
+
+  // tile size and LDS footprint of the generic pass
+  const int64_t Mw = a.bs_M ? a.bs_M : M;  // points per work line
+  a.Mp = Mw > 0 ? (Mw % 2 == 0 ? Mw + 1 : Mw) : 0;
+  auto footprint = [&](int64_t T, int64_t* w1off) {
+    const int64_t Tw = a.r2r_cplx ? 2 * T : T;
+    const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
+    const int64_t w1 = (lk == L_COPY || a.nstages == 0) ? 0 : Tw * a.Mp * cs;
+    const int64_t raw = T * a.raw_pitch * rs;
+    *w1off = (w0 + 255) / 256 * 256;
+    return *w1off + std::max(w1, raw);
+  };
+  int64_t T = 1;
+  while (T < 64 && T < a.ldim[0]) T *= 2;
+  int64_t off = 0;
+  int64_t bytes = footprint(T, &off);
+  while (bytes > 64 * 1024 && T > 1) {
+    T /= 2;
+    bytes = footprint(T, &off);
+  }
+  if (bytes > kMaxLds) {
+    *fits = false;
+    return PFFT_OK;
+  }
+  a.T = (int32_t)T;
+  a.lds_w1_off = off;
+  pp.lds_bytes = bytes;
+  pp.threads = kBlockThreads;
+  int64_t nt = (a.ldim[0] + T - 1) / T;
+  for (int i = 1; i < a.D; ++i) nt *= a.ldim[i];
+  a.ntiles = nt;
+  pp.grid = std::min<int64_t>(nt, 1 << 20);
+  a.scale = 1.0;
+  a.do_scale = 0;
+  pp.gen_args = a;
+  pp.gen_threads = pp.threads;
+  pp.gen_lds = pp.lds_bytes;
+  pp.gen_grid = pp.grid;
+  return PFFT_OK;
+}
+
+// Four-step form of a c2c line too long for one LDS-resident pass: M = M1 M2
+// with j = M2 j1 + j2 and k = k1 + M1 k2,
+//   pass A: for each j2, the M1-point FFT over j1, times w_M^(j2 k1), written
+//           to an intermediate Y[j2 + M2 (k1 + M1 r)] (r: the other line dims,
+//           dense);
+//   pass B: for each k1, the M2-point FFT over j2 of Y, written at k.
+// Pass A's line dim 0 is j2 (its store's contiguous axis), which the kernel
+// reads back as the tile's first line index for the twiddle.
+static int split_pass(Plan& p, const PassPlan& whole, const std::vector<LineDim>& lds, int dim,
+                      std::vector<PassPlan>& out) {
+  const PassArgs& w = whole.args;
+  const int64_t M = w.M;
+  int64_t M2 = 0;
+  for (int64_t f = 2; f <= M / 2; ++f)
+    if (M % f == 0 && f * f >= M) { M2 = f; break; }
+  if (!M2 || w.op || w.r2r_cplx)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "line of length " + std::to_string(M) + " does not fit the LDS-resident pass");
+  const int64_t M1 = M / M2;
+  int64_t rest = 1;
+  std::vector<int64_t> R;
+  for (const LineDim& d : lds) {
+    R.push_back(rest);
+    rest *= d.len;
+  }
+  PassPlan A = whole, B = whole;
+  for (PassPlan* q : {&A, &B}) {
+    q->tw_id = q->aux_id = q->bs_chirp_id = q->bs_bhat_id = q->fs_tw_id = -1;
+    PassArgs& a = q->args;
+    a.bs_M = 0;
+    a.fs_M = 0;
+  }
+  const int64_t mid = M * rest;   // intermediate elements
+  // pass A
+  {
+    PassArgs& a = A.args;
+    a.n = a.M = a.L_in = a.L_out = M1;
+    a.sk_in = M2 * w.sk_in;
+    a.sk_out = M2;
+    a.raw_pitch = 2 * M1 + 2;
+    a.fs_M = M;
+    A.fs_tw_id = twiddle_table(p, M);
+    A.out_elems = mid;
+    A.elt_out = whole.elt_in;
+    std::vector<LineDim> la = {{M2, w.sk_in, 1, -1}};
+    for (size_t i = 0; i < lds.size(); ++i) la.push_back({lds[i].len, lds[i].si, M * R[i], lds[i].logical});
+    int r = line_fft(p, A);
+    bool fits = true;
+    if (!r) r = line_layout(p, A, la, dim, &fits);
+    if (r) return r;
+    if (!fits || A.args.ldim[0] != M2)
+      return fail(PFFT_ERR_UNSUPPORTED,
+                  "line of length " + std::to_string(M) + " does not fit the LDS-resident pass");
+  }
+  // pass B
+  {
+    PassArgs& a = B.args;
+    a.n = a.M = a.L_in = a.L_out = M2;
+    a.sk_in = 1;
+    a.sk_out = M1 * w.sk_out;
+    a.raw_pitch = 2 * M2 + 2;
+    B.in_elems = mid;
+    B.elt_in = whole.elt_out;
+    std::vector<LineDim> lb = {{M1, M2, w.sk_out, -1}};
+    for (size_t i = 0; i < lds.size(); ++i) lb.push_back({lds[i].len, M * R[i], lds[i].so, lds[i].logical});
+    int r = line_fft(p, B);
+    bool fits = true;
+    if (!r) r = line_layout(p, B, lb, dim, &fits);
+    if (r) return r;
+    if (!fits)
+      return fail(PFFT_ERR_UNSUPPORTED,
+                  "line of length " + std::to_string(M) + " does not fit the LDS-resident pass");
+  }
+  A.split_part = 1;
+  B.split_part = 2;
+  out.push_back(A);
+  out.push_back(B);
+  return PFFT_OK;
+}
+
 static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t elt_out,
                       const int32_t* perm_in, const int64_t* dims_in, const int32_t* perm_out,
-                      const int64_t* dims_out, PassPlan& pp, int64_t pad_in = 0,
-                      int64_t pad_out = 0) {
+                      const int64_t* dims_out, std::vector<PassPlan>& out,
+                      int64_t pad_in = 0, int64_t pad_out = 0) {
   const int N = p.N;
+  PassPlan pp;
   PassArgs& a = pp.args;
   memset(&a, 0, sizeof(a));
   pp.elt_in = elt_in;
@@ -397,115 +673,33 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
   a.n = n;
   a.M = M;
 
-  if (lk != L_COPY) {
-    const int ns = radix_schedule(M, a.radix);
-    if (ns < 0) return fail(PFFT_ERR_UNSUPPORTED, "too many radix stages");
-    a.nstages = ns;
-    pp.tw_id = twiddle_table(p, M);
-    // aux tables
-    std::vector<double> aux;
-    if (lk == L_R2C_HALF || lk == L_C2R_HALF) {
-      aux.resize(2 * (M + 1));
-      for (int64_t k = 0; k <= M; ++k) expm(k, n, &aux[2 * k], &aux[2 * k + 1]);
-    } else if (lk == L_DCT2 || lk == L_DST2 || lk == L_DCT3 || lk == L_DST3) {
-      aux.resize(2 * n);
-      for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
-    } else if (lk == L_DCT4 || lk == L_DST4) {
-      aux.resize(4 * n);
-      for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
-      for (int64_t k = 0; k < n; ++k)
-        expm(2 * k + 1, 8 * n, &aux[2 * (n + k)], &aux[2 * (n + k) + 1]);
-    } else if (lk == L_C2C && M == 1024 && p.real_type == PFFT_F32) {
-      // middle twiddles of the 32 x 32 MFMA form of 1024-point lines, in the
-      // order its lanes read them: entry 64 i + l is w1024^(k2 n1) with
-      // k2 = l mod 32 and n1 = (i mod 4) + 8 (i div 4) + 4 (l div 32), the
-      // row of accumulator register i of lane l
-      aux.resize(2 * 1024);
-      for (int i = 0; i < 16; ++i)
-        for (int l = 0; l < 64; ++l) {
-          const int64_t k2 = l & 31, n1 = (i & 3) + 8 * (i >> 2) + 4 * (l >> 5);
-          expm(k2 * n1, 1024, &aux[2 * (64 * i + l)], &aux[2 * (64 * i + l) + 1]);
-        }
-    }
-    if (!aux.empty()) pp.aux_id = add_table(p, "aux", std::move(aux));
-  }
-
+  const int comp_in = a.in_cplx ? 2 : 1;
+  a.raw_pitch = comp_in * a.L_in + comp_in;
   // line dims: every logical dim but `dim`, plus the flattened extra dim
-  struct LD { int64_t len, si, so; int logical; };
+  using LD = LineDim;
   std::vector<LD> lds;
   for (int d = 0; d < N; ++d)
     if (d != dim) lds.push_back({dims_in[d], sin_[d], sout[d], d});
   if (p.E > 1) lds.push_back({p.E, sin_e, sout_e, N});
   if (lds.empty()) lds.push_back({1, 0, 0, -1});
+  if (lk == L_C2C) {
+    // one line of the direct form: two work buffers of M + 1 points
+    const int64_t cs = p.real_type == PFFT_F32 ? 8 : 16;
+    const int64_t w0 = ((M + 1) * cs + 255) / 256 * 256;
+    if (w0 + (M + 1) * cs > kMaxLds) return split_pass(p, pp, lds, dim, out);
+  }
+  if (lk != L_COPY) {
+    const int r = line_fft(p, pp);
+    if (r) return r;
+  }
 
-  // Line dims in input-memory order; then the tile axis goes first: the
-  // output's contiguous axis when the store transposes, else the input's
-  // contiguous axis when the load is strided, else the line dim nearest in
-  // input memory (consecutive lines are then adjacent in memory).
-  std::stable_sort(lds.begin(), lds.end(), [](const LD& x, const LD& y) { return x.si < y.si; });
-  int tile = -1;
-  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
-    if (a.sk_out != 1 && lds[i].so == 1 && lds[i].len > 1) tile = (int)i;
-  for (size_t i = 0; i < lds.size() && tile < 0; ++i)
-    if (a.sk_in != 1 && lds[i].si == 1 && lds[i].len > 1) tile = (int)i;
-  if (tile > 0) {
-    const LD t = lds[tile];
-    lds.erase(lds.begin() + tile);
-    lds.insert(lds.begin(), t);
-  }
-  // drop unit dims except the first
-  std::vector<LD> keep;
-  for (size_t i = 0; i < lds.size(); ++i)
-    if (i == 0 || lds[i].len > 1) keep.push_back(lds[i]);
-  if ((int)keep.size() > kMaxDims) return fail(PFFT_ERR_UNSUPPORTED, "too many dimensions");
-  a.D = (int32_t)keep.size();
-  for (int i = 0; i < a.D; ++i) {
-    a.ldim[i] = keep[i].len;
-    a.lsin[i] = keep[i].si;
-    a.lsout[i] = keep[i].so;
-    a.llog[i] = (keep[i].logical >= 0 && keep[i].logical < N) ? keep[i].logical : -1;
-  }
-  a.pdim = dim;
-
-  // tile size and LDS footprint of the generic pass
-  const int64_t rs = p.real_type == PFFT_F32 ? 4 : 8;
-  const int64_t cs = 2 * rs;
-  const int comp_in = a.in_cplx ? 2 : 1;
-  a.raw_pitch = comp_in * a.L_in + comp_in;
-  a.Mp = M > 0 ? (M % 2 == 0 ? M + 1 : M) : 0;
-  auto footprint = [&](int64_t T, int64_t* w1off) {
-    const int64_t Tw = a.r2r_cplx ? 2 * T : T;
-    const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
-    const int64_t w1 = (lk == L_COPY || a.nstages == 0) ? 0 : Tw * a.Mp * cs;
-    const int64_t raw = T * a.raw_pitch * rs;
-    *w1off = (w0 + 255) / 256 * 256;
-    return *w1off + std::max(w1, raw);
-  };
-  int64_t T = 1;
-  while (T < 64 && T < a.ldim[0]) T *= 2;
-  int64_t off = 0;
-  int64_t bytes = footprint(T, &off);
-  while (bytes > 64 * 1024 && T > 1) {
-    T /= 2;
-    bytes = footprint(T, &off);
-  }
-  if (bytes > 160 * 1024)
+  bool fits = true;
+  int r = line_layout(p, pp, lds, dim, &fits);
+  if (r) return r;
+  if (!fits)
     return fail(PFFT_ERR_UNSUPPORTED,
                 "line of length " + std::to_string(n) + " does not fit the LDS-resident pass");
-  a.T = (int32_t)T;
-  a.lds_w1_off = off;
-  pp.lds_bytes = bytes;
-  pp.threads = kBlockThreads;
-  int64_t nt = (a.ldim[0] + T - 1) / T;
-  for (int i = 1; i < a.D; ++i) nt *= a.ldim[i];
-  a.ntiles = nt;
-  pp.grid = std::min<int64_t>(nt, 1 << 20);
-  a.scale = 1.0;
-  a.do_scale = 0;
-  pp.gen_args = a;
-  pp.gen_threads = pp.threads;
-  pp.gen_lds = pp.lds_bytes;
-  pp.gen_grid = pp.grid;
+  out.push_back(pp);
   return PFFT_OK;
 }
 
@@ -535,6 +729,9 @@ int op_setup(const Plan& p, int dir, const pfft_spectral_op& op, int* pass, int*
     return fail(PFFT_ERR_UNSUPPORTED,
                 "the pass carrying a spectral operator must be a c2c one (two or more "
                 "transformed dimensions)");
+  if (list[*pass].split_part)
+    return fail(PFFT_ERR_UNSUPPORTED,
+                "a spectral operator cannot ride on a line split into two passes");
   return PFFT_OK;
 }
 
@@ -710,11 +907,9 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
       if (n == N - 1) memcpy(outp, s.perm, sizeof(outp));
       else choose(cur, p.stages[n + 1].perm, s.dims_after, outp);
       const int64_t pad_out = n == N - 1 ? 0 : work_pad(outp, s.dims_after, s.elt_out, n + 1);
-      PassPlan pp;
       int r = build_pass(p, n, s.kind_fwd, s.elt_in, s.elt_out, cur, s.dims_before, outp,
-                         s.dims_after, pp, cur_pad, pad_out);
+                         s.dims_after, p.fwd, cur_pad, pad_out);
       if (r) return r;
-      p.fwd.push_back(pp);
       memcpy(cur, outp, sizeof(cur));
       cur_pad = pad_out;
     }
@@ -729,11 +924,9 @@ int plan_build(Plan& p, int32_t N, const int64_t* dims, const int32_t* kinds,
       if (n == 0) memcpy(outp, s.perm, sizeof(outp));
       else choose(cur, p.stages[n - 1].perm, s.dims_before, outp);
       const int64_t pad_out = n == 0 ? pad_real : work_pad(outp, s.dims_before, s.elt_in, n - 1);
-      PassPlan pp;
       int r = build_pass(p, n, s.kind_bwd, s.elt_out, s.elt_in, cur, s.dims_after, outp,
-                         s.dims_before, pp, cur_pad, pad_out);
+                         s.dims_before, p.bwd, cur_pad, pad_out);
       if (r) return r;
-      p.bwd.push_back(pp);
       memcpy(cur, outp, sizeof(cur));
       cur_pad = pad_out;
     }

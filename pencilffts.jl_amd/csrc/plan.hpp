@@ -44,6 +44,10 @@ struct PassPlan {
   int64_t in_elems, out_elems;
   int32_t tw_id;            // index into Plan::tables (twiddles), -1 = none
   int32_t aux_id;           // index into Plan::tables (aux), -1 = none
+  int32_t bs_chirp_id = -1; // Bluestein chirp / kernel spectrum tables, -1 = none
+  int32_t bs_bhat_id = -1;
+  int32_t fs_tw_id = -1;    // four-step twiddles, -1 = none
+  int32_t split_part = 0;   // 1 / 2: first / second pass of a four-step split
   int32_t variant;          // kernel family chosen (0 = generic LDS pass)
   int32_t threads;
   int64_t lds_bytes;
@@ -99,6 +103,20 @@ struct Plan {
   size_t in_bytes = 0, out_bytes = 0, work_bytes = 0, work0_bytes = 0;
   DeviceState* dev = nullptr;
 };
+
+// Points a pass descriptor at the plan's tables: the device copies, or (host)
+// the copies rounded to the plan precision that the emulator reads.
+inline void bind_tables(const Plan& p, const PassPlan& pp, PassArgs& a, bool host = false) {
+  auto at = [&](int32_t id) -> const void* {
+    if (id < 0) return nullptr;
+    return host ? (const void*)p.tables[id].host.data() : (const void*)p.tables[id].dev;
+  };
+  a.tw = at(pp.tw_id);
+  a.aux = at(pp.aux_id);
+  a.bs_chirp = at(pp.bs_chirp_id);
+  a.bs_bhat = at(pp.bs_bhat_id);
+  a.fs_tw = at(pp.fs_tw_id);
+}
 
 int plan_build(Plan& p, int32_t ndims, const int64_t* dims, const int32_t* kinds,
                int32_t real_type, int32_t permute, int32_t n_extra,

@@ -60,6 +60,24 @@ class Emulator:
         finally:
             self.lib.pfft_emu_plan_destroy(h)
 
+    def bluestein(self, dims, kinds, real_type, permute=True):
+        """Convolution length of the Bluestein form of each forward pass (0:
+        direct radix stages)."""
+        N = len(dims)
+        h = C.c_void_p()
+        st = self.lib.pfft_emu_plan_create(
+            C.byref(h), N, (C.c_int64 * N)(*dims), (C.c_int32 * N)(*kinds), real_type,
+            int(permute), 0, (C.c_int64 * 1)(), 0)
+        if st:
+            raise RuntimeError(self.lib.pfft_emu_last_error().decode())
+        try:
+            out = (C.c_int64 * 16)()
+            self.lib.pfft_emu_bluestein.restype = C.c_int
+            n = self.lib.pfft_emu_bluestein(h, out, 16)
+            return [out[i] for i in range(n)]
+        finally:
+            self.lib.pfft_emu_plan_destroy(h)
+
     def run_inplace(self, dims, kinds, real_type, directions, buf, permute=True, extra=()):
         """In-place plan: runs each direction in turn on the uint8 buffer `buf`
         (input and output views share it); returns the plan's buffer size."""

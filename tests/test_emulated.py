@@ -161,3 +161,64 @@ def test_padded_work_buffers(pf, emu, dims, trs_name, real_type):
     ref = OP.memory_image(OP.forward(meta, OP.from_image(xw, meta.mem_dims("in"))))
     err = OP.rel_l2(got[0][0].view(NP_DTYPES[meta.eltype_out]), ref)
     assert err <= tol_for(CODE_NAMES[real_type], dims), err
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims,trs_name,bs", [
+    ((1031, 6), "FFT", [4096, 0]),        # prime line: Bluestein at 4096 points
+    ((2062, 4), "RFFT", [4096, 0]),       # 1031-point half-length FFT
+    ((8, 134), "FFT", [0, 512]),          # 2 x 67 along dim 2
+    ((67, 5), "R2R:REDFT10", [256, 0]), ((67, 5), "R2R:REDFT01", [256, 0]),
+    ((67, 5), "R2R:REDFT11", [512, 0]),   # 134-point FFT
+    ((66, 5), "R2R:RODFT00", [512, 0]),   # 134-point odd extension
+    ((67, 5), "R2R:DHT", [256, 0]), ((67, 5), "R2R:RODFT10", [256, 0]),
+    ((71, 9), "R2R:REDFT00", [0, 0]),     # 140 = 4 * 5 * 7: direct stages
+])
+def test_bluestein_lengths(pf, emu, dims, trs_name, bs, real_type):
+    """Lines whose length has a prime factor above 64 run the generic pass in
+    its Bluestein (chirp-z) form: the oracle's direct transforms at those
+    lengths, forward and back (the reference accepts any length, e.g. the
+    odd sizes of test/rfft.jl)."""
+    T = pf.Transforms
+    if trs_name.startswith("R2R:"):
+        trs = (T.R2R(getattr(T, trs_name[4:])),)
+    else:
+        trs = (getattr(T, trs_name)(),)
+    err_f, err_b = emu_case(pf, emu, dims, trs, real_type)
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    assert err_f <= tol and err_b <= tol, (err_f, err_b, tol)
+    _, full = meta_for(T, dims, trs, CODE_NAMES[real_type], True, ())
+    assert emu.bluestein(dims, [t.code for t in full], real_type) == bs
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims,trs_name,npass", [
+    ((16384, 3), "FFT", 3),         # 128 x 128 four-step split, then the 3-point pass
+    ((5, 16384), "FFT", 3),         # split on the second (permuted) dim
+    ((6, 20000), "RFFT", 3),        # r2c on dim 1, 20000 = 125 x 160 c2c split on dim 2
+    ((12288, 2), "BFFT", 3),        # 96 x 128
+])
+def test_four_step_lengths(pf, emu, dims, trs_name, npass, real_type):
+    """c2c lines too long for one LDS-resident pass run as two passes (the
+    four-step split M = M1 M2, twiddle w_M^(j2 k1) between them): the
+    oracle's transforms at those lengths, forward and back."""
+    T = pf.Transforms
+    trs = (getattr(T, trs_name)(),)
+    err_f, err_b = emu_case(pf, emu, dims, trs, real_type)
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    assert err_f <= tol and err_b <= tol, (err_f, err_b, tol)
+    _, full = meta_for(T, dims, trs, CODE_NAMES[real_type], True, ())
+    assert len(emu.bluestein(dims, [t.code for t in full], real_type)) == npass
+
+
+def test_four_step_with_bluestein(pf, emu):
+    """64 x 1031: the split's second pass runs its 1031-point lines in the
+    Bluestein form."""
+    T = pf.Transforms
+    dims = (64 * 1031, 2)
+    for rt in (1, 0):
+        err_f, err_b = emu_case(pf, emu, dims, (T.FFT(),), rt)
+        tol = tol_for(CODE_NAMES[rt], dims)
+        assert err_f <= tol and err_b <= tol, (rt, err_f, err_b)
+        _, full = meta_for(T, dims, (T.FFT(),), CODE_NAMES[rt], True, ())
+        assert emu.bluestein(dims, [t.code for t in full], rt) == [0, 4096, 0]
