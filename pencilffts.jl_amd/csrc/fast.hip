@@ -155,12 +155,17 @@ __device__ __forceinline__ void dft(C<R>* v) {
 // Lane l owns butterflies j = l + L i.  With FROM_REG the inputs come from
 // the strided registers v (index i + BPL r); with TO_REG (last stage, Ns RAD
 // == M) the outputs land in v in the same strided order.
-template <typename R, int M, int L, int P, int RAD, int NS, bool FROM_REG, bool TO_REG>
+template <typename R, int M, int L, int P, int RAD, int NS, bool FROM_REG, bool TO_REG,
+          bool BFLY_REG = false>
 __device__ __forceinline__ void stage(C<R>* v, C<R>* line, int l, const C<R>* __restrict__ tw) {
   constexpr int NB = M / RAD;
   constexpr int BPL = NB / L;
   static_assert(BPL * RAD == P, "radix must divide the points per lane");
-  static_assert(!TO_REG || NS * RAD == M, "registers hold the output of the last stage only");
+  // TO_REG leaves output r of the lane's butterfly i in register i + BPL r:
+  // the natural strided order after the last stage; BFLY_REG asks for that
+  // butterfly order after an inner stage (a cross-lane exchange follows)
+  static_assert(!TO_REG || BFLY_REG || NS * RAD == M,
+                "registers hold the output of the last stage only");
   C<R> u[P];
   const C<R>* lp = line + pad8(l);
 #pragma unroll
@@ -213,6 +218,64 @@ __device__ __forceinline__ void fft_rec(C<R>* v, C<R>* line, int l, const C<R>* 
     line_sync<L>();
     fft_rec<R, M, L, P, NS * RAD, false, REST...>(v, line, l, tw);
   }
+}
+
+// ---- 512-point lines with the last exchange across lanes (MF = 4) -------------
+// The Stockham form moves data between stages through the line's LDS slot.
+// Between stages 2 and 3 of 512 = 8 . 8 . 8 (one wavefront per line, 8
+// points per lane) that exchange is a pure digit transpose: point
+// 64 a + 8 b + c sits in lane 8 a + c, register b after stage 2 and is needed
+// in lane 8 b + c, register a by stage 3.  Swapping lane bit 3 + k with
+// register bit k for k = 0, 1, 2 does it in registers: bit 3 by a DPP row
+// rotation by 8 (lane l ^ 8 within a row of 16), bits 4 and 5 by gfx950's
+// v_permlane16_swap / v_permlane32_swap, which exchange a register pair's
+// odd rows (upper half) with the other's even rows (lower half) directly.
+__device__ __forceinline__ void xl_pair_row8(float& r0, float& r1, bool hi) {
+  // lane bit 3 set: keeps r1, sends r0, receives the partner's r1 into r0
+  const float send = hi ? r0 : r1;
+  const float recv = __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128 /* row_ror:8 */, 0xf, 0xf, false));
+  if (hi) r0 = recv;
+  else r1 = recv;
+}
+__device__ __forceinline__ void xl_pair_swap16(float& r0, float& r1) {
+  const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(r0), __float_as_uint(r1),
+                                                  false, false);
+  r0 = __uint_as_float(s[0]);
+  r1 = __uint_as_float(s[1]);
+}
+__device__ __forceinline__ void xl_pair_swap32(float& r0, float& r1) {
+  const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(r0), __float_as_uint(r1),
+                                                  false, false);
+  r0 = __uint_as_float(s[0]);
+  r1 = __uint_as_float(s[1]);
+}
+__device__ __forceinline__ void xl_transpose8(C<float>* v, int l) {
+  const bool hi = (l >> 3) & 1;
+#pragma unroll
+  for (int r = 0; r < 8; r += 2) {
+    xl_pair_row8(v[r].re, v[r + 1].re, hi);
+    xl_pair_row8(v[r].im, v[r + 1].im, hi);
+  }
+#pragma unroll
+  for (int r : {0, 1, 4, 5}) {
+    xl_pair_swap16(v[r].re, v[r + 2].re);
+    xl_pair_swap16(v[r].im, v[r + 2].im);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    xl_pair_swap32(v[r].re, v[r + 4].re);
+    xl_pair_swap32(v[r].im, v[r + 4].im);
+  }
+}
+// v: x[l + 64 p] in register p on entry, X[l + 64 p] on exit (as Cfg<512>)
+__device__ __forceinline__ void fft512_xl(C<float>* v, C<float>* line, int l,
+                                          const C<float>* __restrict__ tw) {
+  stage<float, 512, 64, 8, 8, 1, true, false>(v, line, l, tw);
+  wave_sync();
+  stage<float, 512, 64, 8, 8, 8, false, true, true>(v, line, l, tw);
+  xl_transpose8(v, l);
+  stage<float, 512, 64, 8, 8, 64, true, true>(v, line, l, tw);
 }
 
 // radix schedules: M = L * P, every radix divides P; W wavefronts per line
@@ -1012,7 +1075,8 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
   const bool conj_out = KIND == K_C2C && a.bwd != 0;
   const R sc = a.do_scale ? (R)a.scale : R(1);
-  static_assert(MF == 0 || (M == 1024 && sizeof(R) == 4), "MFMA stages: 1024-point f32 lines");
+  static_assert(MF == 0 || (MF == 4 ? M == 512 && W == 1 : M == 1024) && sizeof(R) == 4,
+                "MFMA stages: 1024-point f32 lines; cross-lane exchange: 512-point f32 lines");
   MfmaA<MF == 1 || MF == 2 ? MF : 1> mfA;
   if constexpr (MF == 1 || MF == 2) mfA.init(l % 64);
   // the 32 x 32 MFMA form's DFT-matrix slices, after the tile
@@ -1222,7 +1286,9 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           }
         }
       }
-      if constexpr (MF != 0) {
+      if constexpr (MF == 4) {
+        fft512_xl((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw);
+      } else if constexpr (MF != 0) {
         int e;
         if constexpr (JM)
           e = fft1024_mfma32((C<float>*)v[q], l, (const C<float>*)aux, ftab);
@@ -1590,6 +1656,19 @@ static void add_size(std::vector<Entry>& v) {
   }
 }
 
+// 512-point FP32 c2c passes with the last stage exchange across lanes
+// (MF = 4): twins of the Stockham variants the tuner picks at 512^3
+static void add_xlane(std::vector<Entry>& v) {
+  add<float, 512, 4, 1, K_C2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, 512, 8, 2, K_C2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, 512, 16, 1, K_C2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 512, 16, 2, K_C2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 512, 16, 1, K_C2C, IO_LINE, IO_TILE, 4, 2>(v);
+  add<float, 512, 16, 2, K_C2C, IO_LINE, IO_TILE, 4, 2>(v);
+  add<float, 512, 16, 1, K_C2C, IO_TILE, IO_LINE, 4>(v);
+  add<float, 512, 16, 2, K_C2C, IO_TILE, IO_LINE, 4>(v);
+}
+
 static const std::vector<Entry>& registry() {
   static std::vector<Entry> v = [] {
     std::vector<Entry> r;
@@ -1632,6 +1711,7 @@ static const std::vector<Entry>& registry() {
     add_op<double, 128>(r);
     add_op<double, 256>(r);
     add_op<double, 512>(r);
+    add_xlane(r);
     return r;
   }();
   return v;
@@ -1764,6 +1844,9 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
     if (p.flags & PFFT_FLAG_MFMA_SPLIT16) want_mf = 3, alt_mf = 1;
     else if (p.flags & PFFT_FLAG_MFMA_BF16X3) want_mf = 2;
   }
+  // 512-point FP32 lines: the cross-lane exchange variants beside the
+  // Stockham ones (same arithmetic, bit-identical results)
+  if (M == 512 && rb == 4) alt_mf = 4;
   const std::vector<Entry>& reg = registry();
   for (size_t i = 0; i < reg.size(); ++i) {
     const Entry& x = reg[i];
@@ -1878,6 +1961,7 @@ const char* fast_name(int variant) {
       (e.mf == 1   ? ",mfma-split16"
        : e.mf == 2 ? ",mfma-bf16x3"
        : e.mf == 3 ? ",mfma32-split16"
+       : e.mf == 4 ? ",xlane"
                    : "") +
       ">";
   return s.c_str();

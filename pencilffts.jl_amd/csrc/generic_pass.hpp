@@ -26,6 +26,36 @@ struct TileCtx {
   int32_t Tw;        // work lines (two per line for complex R2R data)
 };
 
+// Mixed-radix digits of e = e0, e0 + step, e0 + 2 step, ... (the strided
+// loops of the phases): one division per digit up front, then additions with
+// carries, so the loop bodies divide by none of the run-time extents.
+// Digit i < N - 1 runs over [0, rad[i]); the last one is unbounded.
+template <int N>
+struct Digits {
+  uint32_t d[N], rad[N], s[N];
+  PFFT_HD Digits(uint32_t e0, uint32_t step, const uint32_t (&radices)[N > 1 ? N - 1 : 1]) {
+    for (int i = 0; i < N - 1; ++i) {
+      rad[i] = radices[i];
+      d[i] = e0 % rad[i];
+      e0 /= rad[i];
+      s[i] = step % rad[i];
+      step /= rad[i];
+    }
+    d[N - 1] = e0;
+    s[N - 1] = step;
+    rad[N - 1] = 0;
+  }
+  PFFT_HD void next() {
+    uint32_t carry = 0;
+    for (int i = 0; i < N - 1; ++i) {
+      d[i] += s[i] + carry;   // < 2 rad[i]: one subtraction at most
+      carry = d[i] >= rad[i];
+      if (carry) d[i] -= rad[i];
+    }
+    d[N - 1] += s[N - 1] + carry;
+  }
+};
+
 // Tile index -> (block b0 of T lines along line dim 0, linear index `rest`
 // over line dims 1..D-1).  Order 0 walks line dim 0 first (consecutive tiles
 // are neighbouring blocks of one run); order 1 walks the other dims first, so
@@ -147,10 +177,10 @@ PFFT_HD void phase_load(const PassArgs& a, const TileCtx& c, int tid, int nthr,
   const uint32_t Tact = (uint32_t)c.Tact;
   const uint32_t count = Tact * L;
   const bool along = (a.sk_in == 1);
-  for (uint32_t e = tid; e < count; e += nthr) {
-    uint32_t t, j;
-    if (along) { t = e / L; j = e - t * L; }
-    else { j = e / Tact; t = e - j * Tact; }
+  Digits<2> it(tid, nthr, {along ? L : Tact});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t t = along ? it.d[1] : it.d[0];
+    const uint32_t j = along ? it.d[0] : it.d[1];
     const int64_t g = c.in_base + (int64_t)t * a.lsin[0] + (int64_t)j * a.sk_in;
     if (a.in_cplx) {
       const cpx<R> v = ((const cpx<R>*)a.in)[g];
@@ -184,9 +214,10 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
   const int64_t n = a.n;
   const uint32_t count = (uint32_t)c.Tw * M;
   const R zero = R(0);
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / M;
-    const uint32_t j = e - w * M;
+  Digits<2> it(tid, nthr, {M});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t w = it.d[1];
+    const uint32_t j = it.d[0];
     cpx<R> v;
     switch (a.kind) {
       case L_C2C:
@@ -334,10 +365,11 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_
   const uint32_t nb = M / RAD;                 // butterflies per line
   const uint32_t step = (uint32_t)(M / (Ns * RAD));
   const uint32_t count = (uint32_t)c.Tw * nb;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / nb;
-    const uint32_t j = e - w * nb;
-    const uint32_t jj = j % (uint32_t)Ns;
+  // e = (w nb + j), j = jq Ns + jj
+  Digits<3> it(tid, nthr, {(uint32_t)Ns, nb / (uint32_t)Ns});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t jj = it.d[0], jq = it.d[1], w = it.d[2];
+    const uint32_t j = jq * (uint32_t)Ns + jj;
     const cpx<R>* s = src + w * a.Mp;
     cpx<R> v[RAD];
     for (int r = 0; r < RAD; ++r) {
@@ -350,7 +382,7 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_
     else if (RAD == 4) dft4(v);
     else if (RAD == 5) dft5(v);
     else if (RAD == 8) dft8(v);
-    cpx<R>* d = dst + w * a.Mp + (j / (uint32_t)Ns) * (uint32_t)Ns * RAD + jj;
+    cpx<R>* d = dst + w * a.Mp + jq * (uint32_t)Ns * RAD + jj;
     for (int r = 0; r < RAD; ++r) d[r * Ns] = bwd ? conj(v[r]) : v[r];
   }
 }
@@ -364,22 +396,29 @@ PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
   const uint32_t nb = M / RAD;
   const uint32_t step = (uint32_t)(M / (Ns * RAD));
   const uint32_t count = (uint32_t)c.Tw * M;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / M;
-    const uint32_t rem = e - w * M;
-    const uint32_t q = rem / nb;   // output index within the butterfly
-    const uint32_t j = rem - q * nb;
-    const uint32_t jj = j % (uint32_t)Ns;
+  // e = w M + q nb + j, j = jq Ns + jj; q: output index within the butterfly
+  Digits<4> it(tid, nthr, {(uint32_t)Ns, nb / (uint32_t)Ns, (uint32_t)RAD});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t jj = it.d[0], jq = it.d[1], q = it.d[2], w = it.d[3];
+    const uint32_t j = jq * (uint32_t)Ns + jj;
     const cpx<R>* s = src + w * a.Mp;
     cpx<R> acc = mk(R(0), R(0));
+    // twiddle of the stage times the DFT_R kernel entry w_R^(r q): the table
+    // index (r jj step + ((r q) mod RAD) nb) mod M, its two terms advanced
+    // by additions (jj step < M / RAD, q < RAD)
+    uint32_t ta = 0, tb = 0;
     for (int r = 0; r < RAD; ++r) {
       cpx<R> x = s[j + r * nb];
       if (bwd) x = conj(x);
-      // twiddle of the stage times the DFT_R kernel entry w_R^(r q)
-      const uint32_t idx = (r * jj * step + ((uint32_t)(r * q) % RAD) * nb) % M;
+      uint32_t idx = ta + tb * nb;
+      if (idx >= M) idx -= M;
       acc = acc + x * tw[idx];
+      ta += jj * step;
+      if (ta >= M) ta -= M;
+      tb += q;
+      if (tb >= (uint32_t)RAD) tb -= RAD;
     }
-    dst[w * a.Mp + (j / (uint32_t)Ns) * (uint32_t)Ns * RAD + jj + q * Ns] =
+    dst[w * a.Mp + jq * (uint32_t)Ns * RAD + jj + q * Ns] =
         bwd ? conj(acc) : acc;
   }
 }
@@ -425,8 +464,9 @@ PFFT_HD void phase_bs_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr
   const cpx<R>* ch = (const cpx<R>*)a.bs_chirp;
   const uint32_t Mb = (uint32_t)a.bs_M, M = (uint32_t)a.M;
   const uint32_t count = (uint32_t)c.Tw * Mb;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / Mb, j = e - w * Mb;
+  Digits<2> it(tid, nthr, {Mb});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t w = it.d[1], j = it.d[0];
     cpx<R>* x = W + w * a.Mp + j;
     if (j < M) {
       const cpx<R> v = a.bwd ? conj(*x) : *x;
@@ -444,8 +484,9 @@ PFFT_HD void phase_bs_mul(const PassArgs& a, const TileCtx& c, int tid, int nthr
   const cpx<R>* bh = (const cpx<R>*)a.bs_bhat;
   const uint32_t Mb = (uint32_t)a.bs_M;
   const uint32_t count = (uint32_t)c.Tw * Mb;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / Mb, j = e - w * Mb;
+  Digits<2> it(tid, nthr, {Mb});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t w = it.d[1], j = it.d[0];
     cpx<R>* x = W + w * a.Mp + j;
     *x = *x * bh[j];
   }
@@ -457,8 +498,9 @@ PFFT_HD void phase_bs_post(const PassArgs& a, const TileCtx& c, int tid, int nth
   const cpx<R>* ch = (const cpx<R>*)a.bs_chirp;
   const uint32_t M = (uint32_t)a.M;
   const uint32_t count = (uint32_t)c.Tw * M;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    const uint32_t w = e / M, k = e - w * M;
+  Digits<2> it(tid, nthr, {M});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t w = it.d[1], k = it.d[0];
     cpx<R>* x = W + w * a.Mp + k;
     const cpx<R> v = *x * ch[k];
     *x = a.bwd ? conj(v) : v;
@@ -560,10 +602,10 @@ PFFT_HD void phase_post(const PassArgs& a, const TileCtx& c, int tid, int nthr,
   const uint32_t count = Tact * L;
   const bool along = (a.sk_out == 1);
   const R sc = (R)a.scale;
-  for (uint32_t e = tid; e < count; e += nthr) {
-    uint32_t t, k;
-    if (along) { t = e / L; k = e - t * L; }
-    else { k = e / Tact; t = e - k * Tact; }
+  Digits<2> it(tid, nthr, {along ? L : Tact});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t t = along ? it.d[1] : it.d[0];
+    const uint32_t k = along ? it.d[0] : it.d[1];
     const int64_t g = c.out_base + (int64_t)t * a.lsout[0] + (int64_t)k * a.sk_out;
     if (a.out_cplx) {
       cpx<R> y;

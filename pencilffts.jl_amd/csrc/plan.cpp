@@ -441,6 +441,12 @@ struct LineDim {
   int logical;
 };
 
+// LDS budget of one generic-pass tile (the tile size T halves until it fits)
+#ifndef PFFT_GENERIC_TILE_LDS
+#define PFFT_GENERIC_TILE_LDS (64 * 1024)
+#endif
+constexpr int64_t kGenericTileLds = PFFT_GENERIC_TILE_LDS;
+
 // Line-dim order, tile size and LDS footprint of the generic pass (`dim`:
 // the logical dim transformed).  *fits = false when one line does not fit
 // the LDS.
@@ -497,7 +503,7 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   while (T < 64 && T < a.ldim[0]) T *= 2;
   int64_t off = 0;
   int64_t bytes = footprint(T, &off);
-  while (bytes > 64 * 1024 && T > 1) {
+  while (bytes > kGenericTileLds && T > 1) {
     T /= 2;
     bytes = footprint(T, &off);
   }

@@ -83,3 +83,43 @@ def test_every_candidate_tiles_reversed(pf, gpu, case):
         assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
     assert any(",rev" in x for x in seen), seen
     assert any(",nt" in x for x in seen), seen
+
+
+def test_xlane_512_bit_identical(pf, gpu):
+    """The 512-point kernels whose last stage exchange runs across lanes
+    (DPP row rotation and the gfx950 row-swap instructions, `xlane` in the
+    name) do the Stockham form's arithmetic in another data movement: every
+    forced candidate of a (512, 64, 32) and a (32, 64, 512) FFT plan gives the
+    bit-identical spectrum, and the candidates cover the cross-lane form of
+    every 512-point pass shape the plans use."""
+    T = pf.Transforms
+    L = pf._lib
+    seen = set()
+    for dims in ((512, 64, 32), (32, 64, 512)):
+        rng = np.random.default_rng(5)
+        x = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(np.complex64)
+        ref = None
+        for k in range(48):
+            plan = pf.PencilFFTPlan(dims, T.FFT(), (1, 1), None, T.Float32,
+                                    flags=L.FLAG_CANDIDATE(k))
+            u = pf.allocate_input(plan)
+            u.logical().copy_(torch.from_numpy(x))
+            v = plan * u
+            torch.cuda.synchronize()
+            out = v.parent.cpu().numpy()
+            seen.update(plan.kernels(0) + plan.kernels(2))
+            if ref is None:
+                ref = out
+                # and the spectrum itself against numpy's (float64) FFT
+                want = np.fft.fftn(x.astype(np.complex128))
+                got = v.logical().cpu().numpy()
+                err = np.linalg.norm(got - want) / np.linalg.norm(want)
+                assert err <= 5e-7 * np.log2(x.size), err
+            else:
+                assert np.array_equal(out, ref), (dims, k, plan.kernels(0))
+    xl = sorted(s for s in seen if "xlane" in s)
+    print(xl)
+    for form in ("line->line", "line->tile", "tile->line"):
+        if any(form in s and ",512,c2c," in s for s in seen):
+            assert any(form in s for s in xl), (form, xl)
+    assert any("line->line" in s for s in xl) and any("line->tile" in s for s in xl), xl
