@@ -277,6 +277,65 @@ __device__ __forceinline__ void fft512_xl(C<float>* v, C<float>* line, int l,
   xl_transpose8(v, l);
   stage<float, 512, 64, 8, 8, 64, true, true>(v, line, l, tw);
 }
+// 256 = 8 . 8 . 4 with 32 lanes per line (two lines per wavefront): after
+// stage 2, point 64 a + 8 b + c (a < 4) sits in lane 8 a + c, register b;
+// stage 3 (radix 4, two butterflies per lane) wants it in lane
+// 8 (b mod 4) + c, register (b div 4) + 2 a.  Lane bits 3 and 4 swap with
+// register bits 0 and 1 (DPP rotation by 8, v_permlane16_swap, both within
+// the line's 32 lanes), then a compile-time renaming of the registers
+// (a0 a1 b2 -> b2 a0 a1) finishes it.
+__device__ __forceinline__ void fft256_xl(C<float>* v, C<float>* line, int l,
+                                          const C<float>* __restrict__ tw) {
+  stage<float, 256, 32, 8, 8, 1, true, false>(v, line, l, tw);
+  wave_sync();
+  stage<float, 256, 32, 8, 8, 8, false, true, true>(v, line, l, tw);
+  const bool hi = (l >> 3) & 1;
+#pragma unroll
+  for (int r = 0; r < 8; r += 2) {
+    xl_pair_row8(v[r].re, v[r + 1].re, hi);
+    xl_pair_row8(v[r].im, v[r + 1].im, hi);
+  }
+#pragma unroll
+  for (int r : {0, 1, 4, 5}) {
+    xl_pair_swap16(v[r].re, v[r + 2].re);
+    xl_pair_swap16(v[r].im, v[r + 2].im);
+  }
+  C<float> u[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) u[((r << 1) & 6) | (r >> 2)] = v[r];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = u[r];
+  stage<float, 256, 32, 8, 4, 64, true, true>(v, line, l, tw);
+}
+// 1024 = 16 . 16 . 4, one wavefront per line, 16 points per lane: after
+// stage 2, point 256 a + 16 b + c (a < 4, c < 16) sits in lane 16 a + c,
+// register b; stage 3 (radix 4, four butterflies per lane) wants it in lane
+// 16 (b mod 4) + c, register (b div 4) + 4 a.  Lane bits 4 and 5 swap with
+// register bits 0 and 1 by the two row-swap instructions alone (no DPP,
+// no selects), then the registers are renamed (a0 a1 b2 b3 -> b2 b3 a0 a1).
+__device__ __forceinline__ void fft1024_xl(C<float>* v, C<float>* line, int l,
+                                           const C<float>* __restrict__ tw) {
+  stage<float, 1024, 64, 16, 16, 1, true, false>(v, line, l, tw);
+  wave_sync();
+  stage<float, 1024, 64, 16, 16, 16, false, true, true>(v, line, l, tw);
+#pragma unroll
+  for (int r = 0; r < 16; r += 2) {   // lane bit 4 <-> register bit 0
+    xl_pair_swap16(v[r].re, v[r + 1].re);
+    xl_pair_swap16(v[r].im, v[r + 1].im);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r)        // lane bit 5 <-> register bit 1
+    if ((r & 2) == 0) {
+      xl_pair_swap32(v[r].re, v[r + 2].re);
+      xl_pair_swap32(v[r].im, v[r + 2].im);
+    }
+  C<float> u[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) u[((r & 3) << 2) | (r >> 2)] = v[r];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = u[r];
+  stage<float, 1024, 64, 16, 4, 256, true, true>(v, line, l, tw);
+}
 
 // radix schedules: M = L * P, every radix divides P; W wavefronts per line
 template <int M, int W = 1> struct Cfg;
@@ -1055,7 +1114,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
     if constexpr (ADDT) return tile[tile_idx<R, T, SW>(L - l, t) + (M - L * (p + 1)) * T];
     else return tile[tile_idx<R, T, SW>(M - l - L * p, t)];
   };
-  static_assert(PF == 0 || (MF == 0 && L <= 64 && OPK == 0 &&
+  static_assert(PF == 0 || ((MF == 0 || MF == 4) && L <= 64 && OPK == 0 &&
                             (LD == IO_TILE ? KIND == K_C2C || KIND == K_C2R
                                            : KIND == K_C2C || KIND == K_R2C)),
                 "prefetching variants: c2c, c2r with transposed loads, r2c with line loads");
@@ -1075,8 +1134,10 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
   const bool conj_out = KIND == K_C2C && a.bwd != 0;
   const R sc = a.do_scale ? (R)a.scale : R(1);
-  static_assert(MF == 0 || (MF == 4 ? M == 512 && W == 1 : M == 1024) && sizeof(R) == 4,
-                "MFMA stages: 1024-point f32 lines; cross-lane exchange: 512-point f32 lines");
+  static_assert(MF == 0 || (MF == 4 ? (M == 256 || M == 512 || M == 1024) && W == 1 : M == 1024) &&
+                               sizeof(R) == 4,
+                "MFMA stages: 1024-point f32 lines; cross-lane exchange: 256/512/1024-point "
+                "f32 lines, one wavefront per line");
   MfmaA<MF == 1 || MF == 2 ? MF : 1> mfA;
   if constexpr (MF == 1 || MF == 2) mfA.init(l % 64);
   // the 32 x 32 MFMA form's DFT-matrix slices, after the tile
@@ -1286,8 +1347,12 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           }
         }
       }
-      if constexpr (MF == 4) {
+      if constexpr (MF == 4 && M == 512) {
         fft512_xl((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw);
+      } else if constexpr (MF == 4 && M == 1024) {
+        fft1024_xl((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw);
+      } else if constexpr (MF == 4) {
+        fft256_xl((C<float>*)v[q], (C<float>*)line, l, (const C<float>*)tw);
       } else if constexpr (MF != 0) {
         int e;
         if constexpr (JM)
@@ -1656,8 +1721,8 @@ static void add_size(std::vector<Entry>& v) {
   }
 }
 
-// 512-point FP32 c2c passes with the last stage exchange across lanes
-// (MF = 4): twins of the Stockham variants the tuner picks at 512^3
+// FP32 passes with the last stage exchange across lanes (MF = 4): twins of
+// the Stockham variants the tuner picks at 512^3
 static void add_xlane(std::vector<Entry>& v) {
   add<float, 512, 4, 1, K_C2C, IO_LINE, IO_LINE, 4>(v);
   add<float, 512, 8, 2, K_C2C, IO_LINE, IO_LINE, 4>(v);
@@ -1667,6 +1732,25 @@ static void add_xlane(std::vector<Entry>& v) {
   add<float, 512, 16, 2, K_C2C, IO_LINE, IO_TILE, 4, 2>(v);
   add<float, 512, 16, 1, K_C2C, IO_TILE, IO_LINE, 4>(v);
   add<float, 512, 16, 2, K_C2C, IO_TILE, IO_LINE, 4>(v);
+  // the r2c / c2r passes of 512 reals (256-point lines) and 256-point c2c
+  add<float, 256, 16, 1, K_R2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 256, 16, 2, K_R2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 256, 16, 1, K_C2R, IO_TILE, IO_LINE, 4>(v);
+  add<float, 256, 16, 2, K_C2R, IO_TILE, IO_LINE, 4>(v);
+  add<float, 256, 16, 1, K_C2R, IO_TILE, IO_LINE, 4, 1, 1, 0, 0, 1>(v);   // prefetching
+  add<float, 256, 16, 1, K_R2C, IO_LINE, IO_TILE, 4, 1, 1, 0, 0, 1>(v);
+  add<float, 256, 8, 1, K_R2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, 256, 8, 1, K_C2R, IO_LINE, IO_LINE, 4>(v);
+  add<float, 256, 8, 1, K_C2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, 256, 16, 1, K_C2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 256, 16, 1, K_C2C, IO_TILE, IO_LINE, 4>(v);
+  // 1024-point c2c (the plain FP32 precision baseline): the tuned shapes
+  add<float, 1024, 16, 2, K_C2C, IO_LINE, IO_TILE, 4, 2>(v);
+  add<float, 1024, 16, 1, K_C2C, IO_LINE, IO_TILE, 4, 2>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, 1024, 4, 1, K_C2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, 1024, 16, 2, K_C2C, IO_TILE, IO_LINE, 4>(v);
+  add<float, 1024, 8, 1, K_C2C, IO_TILE, IO_LINE, 4>(v);
 }
 
 static const std::vector<Entry>& registry() {
@@ -1844,9 +1928,10 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
     if (p.flags & PFFT_FLAG_MFMA_SPLIT16) want_mf = 3, alt_mf = 1;
     else if (p.flags & PFFT_FLAG_MFMA_BF16X3) want_mf = 2;
   }
-  // 512-point FP32 lines: the cross-lane exchange variants beside the
-  // Stockham ones (same arithmetic, bit-identical results)
-  if (M == 512 && rb == 4) alt_mf = 4;
+  // 256-, 512- and 1024-point FP32 lines: the cross-lane exchange variants
+  // beside the Stockham ones (same arithmetic, bit-identical results with
+  // the one-wavefront Stockham kernels of the same length)
+  if ((M == 512 || M == 256 || (M == 1024 && want_mf == 0)) && rb == 4) alt_mf = 4;
   const std::vector<Entry>& reg = registry();
   for (size_t i = 0; i < reg.size(); ++i) {
     const Entry& x = reg[i];

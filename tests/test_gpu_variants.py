@@ -85,22 +85,32 @@ def test_every_candidate_tiles_reversed(pf, gpu, case):
     assert any(",nt" in x for x in seen), seen
 
 
-def test_xlane_512_bit_identical(pf, gpu):
-    """The 512-point kernels whose last stage exchange runs across lanes
-    (DPP row rotation and the gfx950 row-swap instructions, `xlane` in the
-    name) do the Stockham form's arithmetic in another data movement: every
-    forced candidate of a (512, 64, 32) and a (32, 64, 512) FFT plan gives the
-    bit-identical spectrum, and the candidates cover the cross-lane form of
-    every 512-point pass shape the plans use."""
+def test_xlane_bit_identical(pf, gpu):
+    """The 256-, 512- and 1024-point kernels whose last stage exchange runs across
+    lanes (DPP row rotation and the gfx950 row-swap instructions, `xlane` in
+    the name) do the Stockham form's arithmetic in another data movement:
+    every forced candidate of each plan below gives the bit-identical output,
+    the first one matches numpy's float64 transform, and the candidates cover
+    the cross-lane form of every 256/512-point pass shape the plans use."""
     T = pf.Transforms
     L = pf._lib
     seen = set()
-    for dims in ((512, 64, 32), (32, 64, 512)):
+    # 1024-point lines also have two-wavefront variants (another radix
+    # schedule, equal to rounding only): those plans are checked per candidate
+    for dims, name in (((512, 64, 32), "FFT"), ((32, 64, 512), "FFT"), ((512, 64, 32), "RFFT"),
+                       ((256, 64, 32), "FFT"), ((32, 32, 256), "FFT"), ((1024, 32, 16), "FFT"),
+                       ((16, 32, 1024), "FFT")):
+        exact = 1024 not in dims
         rng = np.random.default_rng(5)
-        x = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(np.complex64)
+        if name == "RFFT":
+            x = rng.standard_normal(dims).astype(np.float32)
+            want = np.fft.rfftn(x.astype(np.float64), axes=(2, 1, 0))   # r2c along dim 1
+        else:
+            x = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(np.complex64)
+            want = np.fft.fftn(x.astype(np.complex128))
         ref = None
         for k in range(48):
-            plan = pf.PencilFFTPlan(dims, T.FFT(), (1, 1), None, T.Float32,
+            plan = pf.PencilFFTPlan(dims, getattr(T, name)(), (1, 1), None, T.Float32,
                                     flags=L.FLAG_CANDIDATE(k))
             u = pf.allocate_input(plan)
             u.logical().copy_(torch.from_numpy(x))
@@ -108,18 +118,18 @@ def test_xlane_512_bit_identical(pf, gpu):
             torch.cuda.synchronize()
             out = v.parent.cpu().numpy()
             seen.update(plan.kernels(0) + plan.kernels(2))
-            if ref is None:
+            if ref is None or not exact:
                 ref = out
-                # and the spectrum itself against numpy's (float64) FFT
-                want = np.fft.fftn(x.astype(np.complex128))
                 got = v.logical().cpu().numpy()
                 err = np.linalg.norm(got - want) / np.linalg.norm(want)
-                assert err <= 5e-7 * np.log2(x.size), err
+                assert err <= 5e-7 * np.log2(x.size), (dims, name, k, err, plan.kernels(0))
             else:
-                assert np.array_equal(out, ref), (dims, k, plan.kernels(0))
+                assert np.array_equal(out, ref), (dims, name, k, plan.kernels(0))
     xl = sorted(s for s in seen if "xlane" in s)
     print(xl)
-    for form in ("line->line", "line->tile", "tile->line"):
-        if any(form in s and ",512,c2c," in s for s in seen):
-            assert any(form in s for s in xl), (form, xl)
-    assert any("line->line" in s for s in xl) and any("line->tile" in s for s in xl), xl
+    assert any(",1024," in s for s in xl), xl
+    for m in (",512,", ",256,"):
+        for kind in ("c2c", "r2c", "c2r"):
+            for form in ("line->line", "line->tile", "tile->line"):
+                if any(m + kind + "," + form in s for s in seen):
+                    assert any(m + kind + "," + form in s for s in xl), (m, kind, form, xl)
