@@ -56,6 +56,12 @@ struct Digits {
   }
 };
 
+// Work-line slot of point k: one padding slot after every 8 points, so
+// that the butterflies' stride-RAD writes (and the stride-8 ones of the
+// second stage) spread over the LDS banks instead of hitting a few of them.
+// The line pitch Mp covers wp(Mw - 1) + 1 slots.
+PFFT_HD uint32_t wp(uint32_t k) { return k + (k >> 3); }
+
 // Tile index -> (block b0 of T lines along line dim 0, linear index `rest`
 // over line dims 1..D-1).  Order 0 walks line dim 0 first (consecutive tiles
 // are neighbouring blocks of one run); order 1 walks the other dims first, so
@@ -294,7 +300,7 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       default:
         v = mk(zero, zero);
     }
-    W[w * a.Mp + j] = v;
+    W[w * a.Mp + wp(j)] = v;
   }
 }
 
@@ -373,7 +379,7 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_
     const cpx<R>* s = src + w * a.Mp;
     cpx<R> v[RAD];
     for (int r = 0; r < RAD; ++r) {
-      cpx<R> x = s[j + r * nb];
+      cpx<R> x = s[wp(j + r * nb)];
       if (bwd) x = conj(x);
       v[r] = r == 0 ? x : x * tw[r * jj * step];
     }
@@ -382,8 +388,9 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_
     else if (RAD == 4) dft4(v);
     else if (RAD == 5) dft5(v);
     else if (RAD == 8) dft8(v);
-    cpx<R>* d = dst + w * a.Mp + jq * (uint32_t)Ns * RAD + jj;
-    for (int r = 0; r < RAD; ++r) d[r * Ns] = bwd ? conj(v[r]) : v[r];
+    cpx<R>* d = dst + w * a.Mp;
+    const uint32_t ob = jq * (uint32_t)Ns * RAD + jj;
+    for (int r = 0; r < RAD; ++r) d[wp(ob + r * (uint32_t)Ns)] = bwd ? conj(v[r]) : v[r];
   }
 }
 
@@ -408,7 +415,7 @@ PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
     // by additions (jj step < M / RAD, q < RAD)
     uint32_t ta = 0, tb = 0;
     for (int r = 0; r < RAD; ++r) {
-      cpx<R> x = s[j + r * nb];
+      cpx<R> x = s[wp(j + r * nb)];
       if (bwd) x = conj(x);
       uint32_t idx = ta + tb * nb;
       if (idx >= M) idx -= M;
@@ -418,7 +425,7 @@ PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
       tb += q;
       if (tb >= (uint32_t)RAD) tb -= RAD;
     }
-    dst[w * a.Mp + jq * (uint32_t)Ns * RAD + jj + q * Ns] =
+    dst[w * a.Mp + wp(jq * (uint32_t)Ns * RAD + jj + q * (uint32_t)Ns)] =
         bwd ? conj(acc) : acc;
   }
 }
@@ -467,7 +474,7 @@ PFFT_HD void phase_bs_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr
   Digits<2> it(tid, nthr, {Mb});
   for (uint32_t e = tid; e < count; e += nthr, it.next()) {
     const uint32_t w = it.d[1], j = it.d[0];
-    cpx<R>* x = W + w * a.Mp + j;
+    cpx<R>* x = W + w * a.Mp + wp(j);
     if (j < M) {
       const cpx<R> v = a.bwd ? conj(*x) : *x;
       *x = v * ch[j];
@@ -487,7 +494,7 @@ PFFT_HD void phase_bs_mul(const PassArgs& a, const TileCtx& c, int tid, int nthr
   Digits<2> it(tid, nthr, {Mb});
   for (uint32_t e = tid; e < count; e += nthr, it.next()) {
     const uint32_t w = it.d[1], j = it.d[0];
-    cpx<R>* x = W + w * a.Mp + j;
+    cpx<R>* x = W + w * a.Mp + wp(j);
     *x = *x * bh[j];
   }
 }
@@ -501,7 +508,7 @@ PFFT_HD void phase_bs_post(const PassArgs& a, const TileCtx& c, int tid, int nth
   Digits<2> it(tid, nthr, {M});
   for (uint32_t e = tid; e < count; e += nthr, it.next()) {
     const uint32_t w = it.d[1], k = it.d[0];
-    cpx<R>* x = W + w * a.Mp + k;
+    cpx<R>* x = W + w * a.Mp + wp(k);
     const cpx<R> v = *x * ch[k];
     *x = a.bwd ? conj(v) : v;
   }
@@ -534,32 +541,32 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
   const cpx<R>* L = W + w * a.Mp;
   const int64_t n = a.n;
   switch (a.kind) {
-    case L_C2R_HALF: return (k & 1) ? L[k >> 1].im : L[k >> 1].re;
+    case L_C2R_HALF: return (k & 1) ? L[wp((uint32_t)(k >> 1))].im : L[wp((uint32_t)(k >> 1))].re;
     case L_C2R_FULL:
-    case L_DCT1: return L[k].re;
+    case L_DCT1: return L[wp((uint32_t)(k))].re;
     case L_DCT2: {
-      const cpx<R> z = aux[k] * L[k];
+      const cpx<R> z = aux[k] * L[wp((uint32_t)(k))];
       return R(2) * z.re;
     }
     case L_DST2: {
       const int64_t kk = n - 1 - k;
-      const cpx<R> z = aux[kk] * L[kk];
+      const cpx<R> z = aux[kk] * L[wp((uint32_t)(kk))];
       return R(2) * z.re;
     }
     case L_DCT3:
     case L_DST3: {
       const int64_t m = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-      const R y = L[m].re;
+      const R y = L[wp((uint32_t)(m))].re;
       return (a.kind == L_DST3 && (k & 1)) ? -y : y;
     }
     case L_DCT4:
     case L_DST4: {
-      const cpx<R> z = aux[n + k] * L[k];
+      const cpx<R> z = aux[n + k] * L[wp((uint32_t)(k))];
       const R y = R(2) * z.re;
       return (a.kind == L_DST4 && (k & 1)) ? -y : y;
     }
-    case L_DST1: return -L[k + 1].im;
-    case L_DHT: return L[k].re - L[k].im;
+    case L_DST1: return -L[wp((uint32_t)(k + 1))].im;
+    case L_DHT: return L[wp((uint32_t)(k))].re - L[wp((uint32_t)(k))].im;
     default: return R(0);
   }
 }
@@ -576,13 +583,13 @@ PFFT_HD cpx<R> post_cplx(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t
   if (a.kind == L_R2C_HALF) {
     const cpx<R>* aux = (const cpx<R>*)a.aux;
     const int64_t M = a.M;
-    const cpx<R> Z = L[k == M ? 0 : k];
-    const cpx<R> Zc = conj(L[k == 0 ? 0 : M - k]);
+    const cpx<R> Z = L[wp((uint32_t)(k == M ? 0 : k))];
+    const cpx<R> Zc = conj(L[wp((uint32_t)(k == 0 ? 0 : M - k))]);
     const cpx<R> E = scl(Z + Zc, R(0.5));
     const cpx<R> O = scl(mul_mi(Z - Zc), R(0.5));
     y = E + aux[k] * O;
   } else {
-    y = L[k];  // C2C, R2C_FULL
+    y = L[wp((uint32_t)(k))];  // C2C, R2C_FULL
   }
   if (a.adjoint && (a.kind == L_R2C_HALF || a.kind == L_R2C_FULL))
     y = scl(y, R(herm_weight(k, a.n)));

@@ -314,6 +314,15 @@ static void bluestein_tables(Plan& p, int64_t M, int64_t Mb, int32_t* chirp_id,
   *bhat_id = add_table(p, "bs_bhat", std::move(bh));
 }
 
+// LDS pitch (complex slots) of a generic-pass work line of Mw points: one
+// pad slot after every 8 points, odd so that neighbouring lines start on
+// other banks (the kernels' wp())
+static int64_t line_pitch_slots(int64_t Mw) {
+  if (Mw <= 0) return 0;
+  const int64_t span = Mw + (Mw - 1) / 8;
+  return span % 2 == 0 ? span + 1 : span;
+}
+
 static int64_t largest_prime_factor(int64_t m) {
   int64_t best = 1;
   for (int64_t q = 2; q * q <= m; ++q)
@@ -394,8 +403,8 @@ static int line_fft(Plan& p, PassPlan& pp) {
     Mb = 1;
     while (Mb < 2 * M - 1) Mb *= 2;
     const int64_t Tw1 = a.r2r_cplx ? 2 : 1;
-    const int64_t w0 = (Tw1 * (Mb + 1) * cs + 255) / 256 * 256;
-    if (w0 + std::max(Tw1 * (Mb + 1) * cs, a.raw_pitch * rs) > kMaxLds) Mb = 0;
+    const int64_t w0 = (Tw1 * line_pitch_slots(Mb) * cs + 255) / 256 * 256;
+    if (w0 + std::max(Tw1 * line_pitch_slots(Mb) * cs, a.raw_pitch * rs) > kMaxLds) Mb = 0;
   }
   const int ns = radix_schedule(Mb ? Mb : M, a.radix);
   if (ns < 0) return fail(PFFT_ERR_UNSUPPORTED, "too many radix stages");
@@ -490,7 +499,7 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
 
   // tile size and LDS footprint of the generic pass
   const int64_t Mw = a.bs_M ? a.bs_M : M;  // points per work line
-  a.Mp = Mw > 0 ? (Mw % 2 == 0 ? Mw + 1 : Mw) : 0;
+  a.Mp = line_pitch_slots(Mw);
   auto footprint = [&](int64_t T, int64_t* w1off) {
     const int64_t Tw = a.r2r_cplx ? 2 * T : T;
     const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
@@ -691,8 +700,8 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
   if (lk == L_C2C) {
     // one line of the direct form: two work buffers of M + 1 points
     const int64_t cs = p.real_type == PFFT_F32 ? 8 : 16;
-    const int64_t w0 = ((M + 1) * cs + 255) / 256 * 256;
-    if (w0 + (M + 1) * cs > kMaxLds) return split_pass(p, pp, lds, dim, out);
+    const int64_t w0 = (line_pitch_slots(M) * cs + 255) / 256 * 256;
+    if (w0 + line_pitch_slots(M) * cs > kMaxLds) return split_pass(p, pp, lds, dim, out);
   }
   if (lk != L_COPY) {
     const int r = line_fft(p, pp);
