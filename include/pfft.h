@@ -266,6 +266,33 @@ int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* 
                     const void* in, void* out, void* work, size_t work_bytes,
                     void* hip_stream);
 
+/* ---- batches of plans of different shapes ----------------------------------
+ * Transforms of several plans (different shapes, kinds or precisions) in one
+ * call: the collection broadcast of operations.jl:139-152 for fields that do
+ * not share a plan (the reference runs them one after the other).  The
+ * result is that of pfft_execute_op (pfft_execute when op is NULL) for each
+ * item in order on `hip_stream`, except that items of different plans may
+ * run at the same time: they are enqueued on up to four library streams
+ * forked from `hip_stream` by events and joined back into it before the call
+ * returns, so the call may be captured in a hipGraph (the library's streams
+ * and events are created on a device's first batch, which must then not be
+ * under capture).  Items of one plan run in their order on one of those
+ * streams, since they share the plan's workspace when `work` is NULL.  No
+ * item may write memory that another item reads or writes.  Batches on one
+ * device from several host threads at once are serialised by a lock.  An
+ * item that fails stops the enqueueing (the items before it stay enqueued)
+ * and its status is returned.                                              */
+typedef struct {
+  pfft_plan* plan;
+  int32_t direction;              /* PFFT_FORWARD ... PFFT_ADJOINT_BACKWARD */
+  const pfft_spectral_op* op;     /* NULL: no operator */
+  const void* in;
+  void* out;
+  void* work;                     /* NULL: the plan's own workspace */
+  size_t work_bytes;
+} pfft_batch_item;
+int pfft_execute_batch(int32_t n_items, const pfft_batch_item* items, void* hip_stream);
+
 /* Divergence-free projection of the spectrum of a 3-component vector field,
  * in place: u_i <- u_i - k_i (k . u) / |k|^2, with the mean mode (k = 0) left
  * as is -- project_divergence_free! of the reference's Navier-Stokes example

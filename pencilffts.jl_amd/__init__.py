@@ -9,13 +9,13 @@ from . import transforms as Transforms
 from . import autodiff, batched, spectral, workloads
 from ._lib import ArgumentError, DeviceError, PfftError, UnsupportedError
 from .plans import (ManyPencilArray, Pencil, PencilArray, PencilFFTPlan, Val, allocate_input,
-                    allocate_output, bmul_, execute_many, export_wisdom, forget_wisdom,
+                    allocate_output, bmul_, execute_batch, execute_many, export_wisdom, forget_wisdom,
                     import_wisdom, ldiv_, mul_, scale_factor)
 
 __all__ = ["Transforms", "PencilFFTPlan", "PencilArray", "ManyPencilArray", "Pencil",
            "allocate_input",
            "allocate_output", "mul_", "ldiv_", "bmul_", "scale_factor", "Val",
-           "execute_many", "export_wisdom", "import_wisdom", "forget_wisdom",
+           "execute_batch", "execute_many", "export_wisdom", "import_wisdom", "forget_wisdom",
            "ArgumentError", "UnsupportedError", "DeviceError", "PfftError",
            "autodiff", "batched", "spectral", "workloads"]
 

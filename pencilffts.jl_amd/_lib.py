@@ -38,6 +38,13 @@ class SpectralOp(C.Structure):
                 ("accumulate", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    """pfft_batch_item: one transform of a pfft_execute_batch call."""
+    _fields_ = [("plan", C.c_void_p), ("direction", C.c_int32),
+                ("op", C.POINTER(SpectralOp)), ("inp", C.c_void_p), ("out", C.c_void_p),
+                ("work", C.c_void_p), ("work_bytes", C.c_size_t)]
+
+
 # every symbol include/pfft.h declares, with (restype, argtypes)
 i32, i64, u32, sz, dbl, vp, cp = (C.c_int32, C.c_int64, C.c_uint32, C.c_size_t, C.c_double,
                                   C.c_void_p, C.c_char_p)
@@ -73,6 +80,7 @@ SIGNATURES = {
     "pfft_execute_noalloc": (C.c_int, [vp, i32, vp, vp, vp]),
     "pfft_execute_op": (C.c_int, [vp, i32, C.POINTER(SpectralOp), vp, vp, vp, sz, vp]),
     "pfft_execute_many": (C.c_int, [vp, i32, i32, P(vp), P(vp), vp, sz, vp]),
+    "pfft_execute_batch": (C.c_int, [i32, P(BatchItem), vp]),
     "pfft_project_divergence_free": (C.c_int, [vp, vp, vp, vp, P(C.c_double), vp]),
     "pfft_tune_export": (C.c_int, [C.c_char_p, sz, P(sz)]),
     "pfft_tune_import": (C.c_int, [C.c_char_p]),

@@ -6,8 +6,10 @@ Here every field shape becomes one *group*:
 * all vector components of all fields of that shape are one array, whose
   trailing extra dims (PencilFFTPlan's `extra_dims`, plans.jl:164-182) batch
   them, so each pass of the group is one kernel launch;
-* the groups' launches are captured in a single hipGraph, so one graph launch
-  runs the whole batch.
+* all groups' executions go to the library in one call (pfft_execute_batch),
+  which runs the groups on forked streams so that they overlap;
+* those launches are captured in a single hipGraph, so one graph launch runs
+  the whole batch.
 
 `GradientEnergy` evaluates, for periodic vector fields u on [0, 2pi)^3,
 
@@ -35,7 +37,7 @@ import torch
 from . import _lib as L
 from . import spectral as S
 from . import transforms as Tr
-from .plans import PencilArray, PencilFFTPlan
+from .plans import PencilArray, PencilFFTPlan, execute_batch
 
 
 class _Group:
@@ -55,6 +57,19 @@ class _Group:
         self.u, self.uh, self.uhb, self.ub = new_in(), new_out(), new_out(), new_in()
         self.g = [new_in() for _ in range(3)]     # d_j u, j = x, y, z
         self.alpha = 2.0 / p.scale_factor
+
+    def items(self):
+        """The same executions as run(), as pfft_execute_batch items (in
+        order: one plan, so they run in this order)."""
+        p = self.plan
+        it = [(p, L.FORWARD, self.u, self.uh)]
+        for j in range(3):
+            it.append((p, L.BACKWARD_SCALED, self.uh, self.g[j], S.spectral_op(S.DERIVATIVE, j)))
+        for j in range(3):
+            it.append((p, L.ADJOINT_BACKWARD, self.g[j], self.uhb,
+                       S.spectral_op(S.DERIVATIVE, j, self.alpha, j > 0)))
+        it.append((p, L.ADJOINT_FORWARD, self.uhb, self.ub))
+        return it
 
     def run(self):
         """Enqueue loss terms and gradient on the current stream."""
@@ -85,9 +100,12 @@ class GradientEnergy:
     """sum |grad u|^2 of many periodic vector fields of uneven cube sizes, and
     its gradient, through grouped plans and one hipGraph."""
 
-    def __init__(self, edges, real_type=L.F32, device="cuda"):
+    def __init__(self, edges, real_type=L.F32, device="cuda", batch=True):
         self.edges = list(edges)
         self.real_type = real_type
+        # batch: every group's executions in one pfft_execute_batch call, the
+        # groups on forked streams (they overlap); else group after group
+        self.batch = batch
         order = sorted(set(self.edges))
         self.groups = {e: _Group(e, self.edges.count(e), real_type, device) for e in order}
         # field i -> (group, index within group)
@@ -113,8 +131,14 @@ class GradientEnergy:
         return g.ub.parent[j].permute(3, 2, 1, 0)
 
     def run(self):
-        for g in self.groups.values():
-            g.run()
+        if self.batch:
+            items = []
+            for g in self.groups.values():
+                items += g.items()
+            execute_batch(items)
+        else:
+            for g in self.groups.values():
+                g.run()
 
     def loss(self):
         return sum(g.loss() for g in self.groups.values())

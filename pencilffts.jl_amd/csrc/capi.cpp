@@ -276,6 +276,20 @@ int pfft_execute_op(pfft_plan* plan, int32_t direction, const pfft_spectral_op* 
                                    (hipStream_t)hip_stream, op->op ? op : nullptr);)
 }
 
+int pfft_execute_batch(int32_t n_items, const pfft_batch_item* items, void* hip_stream) {
+  if (n_items < 0) return fail(PFFT_ERR_ARGUMENT, "negative item count");
+  if (n_items == 0) return PFFT_OK;
+  NEED(items);
+  PFFT_GUARD({
+    std::vector<Plan*> plans((size_t)n_items);
+    for (int32_t i = 0; i < n_items; ++i) {
+      if (!items[i].plan) return fail(PFFT_ERR_ARGUMENT, "null plan in batch item");
+      plans[i] = &items[i].plan->p;
+    }
+    return device_execute_batch(n_items, plans.data(), items, (hipStream_t)hip_stream);
+  })
+}
+
 int pfft_project_divergence_free(pfft_plan* plan, void* uh_x, void* uh_y, void* uh_z,
                                  const double* scale, void* hip_stream) {
   NEED(plan);

@@ -714,6 +714,64 @@ int device_prepare(Plan& p) {
   return PFFT_OK;
 }
 
+// ---- batches of plans (pfft_execute_batch) -----------------------------------
+// Per device: four streams (default priority) and the fork / join events,
+// created on the device's first batch and kept to the end of the process.
+struct BatchStreams {
+  hipStream_t s[4] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[4] = {};
+};
+static std::mutex g_batch_mu;
+static std::map<int, BatchStreams> g_batch;
+
+int device_execute_batch(int n, Plan* const* plans, const pfft_batch_item* items,
+                         hipStream_t stream) {
+  // the plans in order of first appearance: plan k runs on stream k mod 4
+  std::vector<const Plan*> order;
+  std::vector<int> lane((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    size_t k = 0;
+    while (k < order.size() && order[k] != plans[i]) ++k;
+    if (k == order.size()) order.push_back(plans[i]);
+    lane[i] = (int)(k % 4);
+  }
+  auto run = [&](int i, hipStream_t st) {
+    const pfft_batch_item& it = items[i];
+    const pfft_spectral_op* op = it.op && it.op->op ? it.op : nullptr;
+    return device_execute(*plans[i], it.direction, it.in, it.out, it.work, it.work_bytes, st, op);
+  };
+  if (order.size() == 1) {   // one plan: its items in order on the caller's stream
+    for (int i = 0; i < n; ++i) {
+      const int r = run(i, stream);
+      if (r) return r;
+    }
+    return PFFT_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  BatchStreams& b = g_batch[dev];
+  if (!b.fork) {
+    for (int k = 0; k < 4; ++k) {
+      HIP_TRY(hipStreamCreateWithFlags(&b.s[k], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&b.join[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
+  }
+  const int used = (int)std::min<size_t>(order.size(), 4);
+  HIP_TRY(hipEventRecord(b.fork, stream));
+  for (int k = 0; k < used; ++k) HIP_TRY(hipStreamWaitEvent(b.s[k], b.fork, 0));
+  int rc = PFFT_OK;
+  for (int i = 0; i < n && !rc; ++i) rc = run(i, b.s[lane[i]]);
+  // join every forked stream back, also after a failed item
+  for (int k = 0; k < used; ++k) {
+    HIP_TRY(hipEventRecord(b.join[k], b.s[k]));
+    HIP_TRY(hipStreamWaitEvent(stream, b.join[k], 0));
+  }
+  return rc;
+}
+
 int device_pass_names(Plan& p, int dir, std::string* out) {
   int r = device_prepare(p);
   if (r) return r;

@@ -138,6 +138,9 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
 // pfft_execute with the intermediates in the library's static device scratch
 int device_execute_noalloc(Plan& p, int dir, const void* in, void* out, hipStream_t stream);
 int device_set_timing(Plan& p, bool on);
+// pfft_execute_batch: the plans of the items, in item order
+int device_execute_batch(int n, Plan* const* plans, const pfft_batch_item* items,
+                         hipStream_t stream);
 int device_project(Plan& p, void* const* uh, const double* scale, hipStream_t stream);
 int device_stage_times(Plan& p, int n, float* ms);
 int device_pass_names(Plan& p, int dir, std::string* out);

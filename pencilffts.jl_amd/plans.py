@@ -515,6 +515,43 @@ def forget_wisdom():
     L.check(L.lib.pfft_tune_forget())
 
 
+def execute_batch(items, stream=None):
+    """Transforms of several out-of-place plans, of different shapes, in one
+    C call (pfft_execute_batch): `items` holds (plan, direction, src, dst) or
+    (plan, direction, src, dst, op) tuples, op a _lib.SpectralOp (see
+    spectral.py).  Items of different plans may run at the same time on the
+    library's streams, forked from and joined back into `stream` (the
+    current stream by default); items of one plan run in order."""
+    items = list(items)
+    if not items:
+        return
+    arr = (L.BatchItem * len(items))()
+    keep = []
+    for k, it in enumerate(items):
+        p, direction, src, dst = it[:4]
+        op = it[4] if len(it) > 4 else None
+        if p.inplace:
+            raise L.UnsupportedError(L.ERR_UNSUPPORTED, "execute_batch takes out-of-place plans")
+        if direction in (L.FORWARD, L.ADJOINT_BACKWARD):
+            p._check_arrays(src, dst, p.pencil_input, p.pencil_output, p.eltype_input,
+                            p.eltype_output)
+        else:
+            p._check_arrays(src, dst, p.pencil_output, p.pencil_input, p.eltype_output,
+                            p.eltype_input)
+        w = p._workspace()
+        arr[k].plan = p._h
+        arr[k].direction = direction
+        if op is not None:
+            keep.append(op)
+            arr[k].op = C.pointer(op)
+        arr[k].inp = src.data_ptr()
+        arr[k].out = dst.data_ptr()
+        arr[k].work = w.data_ptr() if w is not None else None
+        arr[k].work_bytes = p.work_bytes
+    s = stream if stream is not None else torch.cuda.current_stream(items[0][2].parent.device)
+    L.check(L.lib.pfft_execute_batch(len(items), arr, s.cuda_stream))
+
+
 def execute_many(direction, p, dsts, srcs):
     """All arrays of a collection in one C call on one stream
     (pfft_execute_many): the batched form of the collection broadcast."""

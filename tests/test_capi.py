@@ -222,6 +222,29 @@ def test_execute_argument_checks_without_gpu(pf):
         pf.mul_([1, 2], plan, [1, 2, 3])
 
 
+def test_execute_batch_argument_checks_without_gpu(pf):
+    """pfft_execute_batch: bad counts and items are refused before any device
+    work; a one-plan batch applies pfft_execute's checks to each item."""
+    L = pf._lib
+    T = pf.Transforms
+    plan = pf.PencilFFTPlan((16, 12, 6), T.FFT(), (1,), None, T.Float64)
+    assert L.lib.pfft_execute_batch(0, None, None) == L.OK
+    assert L.lib.pfft_execute_batch(-1, None, None) == L.ERR_ARGUMENT
+    assert L.lib.pfft_execute_batch(1, None, None) == L.ERR_ARGUMENT
+    items = (L.BatchItem * 2)()
+    items[0].plan = plan._h
+    items[0].direction = L.FORWARD
+    base = 1 << 40
+    items[0].inp, items[0].out = base, base << 1
+    assert L.lib.pfft_execute_batch(2, items, None) == L.ERR_ARGUMENT   # items[1].plan is null
+    assert "null plan" in L.lib.pfft_last_error().decode()
+    items[1] = items[0]
+    items[0].out = base + 64     # the first item aliases its input
+    st = L.lib.pfft_execute_batch(2, items, None)
+    assert st == L.ERR_ARGUMENT
+    assert L.lib.pfft_last_error().decode() == "out-of-place plan applied to aliased data"
+
+
 def test_tune_data_round_trip_and_refusal(pf):
     """pfft_tune_export / pfft_tune_import on the host: the export carries the
     build and device header, imports back, and data of another build, a bad
