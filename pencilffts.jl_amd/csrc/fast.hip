@@ -1753,6 +1753,18 @@ static void add_xlane(std::vector<Entry>& v) {
   add<float, 1024, 8, 1, K_C2C, IO_TILE, IO_LINE, 4>(v);
 }
 
+// Row-chunked transposed stores for the mixed-radix FP32 lengths: a 16-line
+// tile of 320 or 384 rows takes 41-49 KB of LDS beside the line slots, one
+// workgroup per CU; in 2 or 4 chunks it takes a half or a quarter of that
+template <int M>
+static void add_mixed_chunks(std::vector<Entry>& v) {
+  add<float, M, 16, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<float, M, 16, 1, K_C2C, IO_LINE, IO_TILE, 0, 4>(v);
+  add<float, M, 16, 2, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
+}
+
 static const std::vector<Entry>& registry() {
   static std::vector<Entry> v = [] {
     std::vector<Entry> r;
@@ -1796,6 +1808,11 @@ static const std::vector<Entry>& registry() {
     add_op<double, 256>(r);
     add_op<double, 512>(r);
     add_xlane(r);
+    add_mixed_chunks<96>(r);
+    add_mixed_chunks<160>(r);
+    add_mixed_chunks<192>(r);
+    add_mixed_chunks<320>(r);
+    add_mixed_chunks<384>(r);
     return r;
   }();
   return v;

@@ -133,3 +133,47 @@ def test_xlane_bit_identical(pf, gpu):
             for form in ("line->line", "line->tile", "tile->line"):
                 if any(m + kind + "," + form in s for s in seen):
                     assert any(m + kind + "," + form in s for s in xl), (m, kind, form, xl)
+
+
+def test_mixed_radix_row_chunks(pf, gpu):
+    """Row-chunked transposed stores (`S=` in the name) of the mixed-radix
+    lengths 96-384: every forced candidate of each plan within the Float32
+    bound of numpy's float64 transform and within rounding of the first
+    candidate (the chunking changes no arithmetic, but another template
+    instance may contract multiply-adds differently), and the chunked kernels
+    of the c2c and r2c passes are among the candidates."""
+    T = pf.Transforms
+    L = pf._lib
+    seen = set()
+    for dims, name in (((384, 48, 32), "FFT"), ((320, 48, 32), "FFT"), ((640, 32, 16), "RFFT"),
+                       ((384, 32, 16), "RFFT"), ((192, 32, 16), "FFT")):
+        rng = np.random.default_rng(9)
+        if name == "RFFT":
+            x = rng.standard_normal(dims).astype(np.float32)
+            want = np.fft.rfftn(x.astype(np.float64), axes=(2, 1, 0))   # r2c along dim 1
+        else:
+            x = (rng.standard_normal(dims) + 1j * rng.standard_normal(dims)).astype(np.complex64)
+            want = np.fft.fftn(x.astype(np.complex128))
+        tol = 5e-7 * np.log2(x.size)
+        ref = None
+        for k in range(40):
+            plan = pf.PencilFFTPlan(dims, getattr(T, name)(), (1, 1), None, T.Float32,
+                                    flags=L.FLAG_CANDIDATE(k))
+            u = pf.allocate_input(plan)
+            u.logical().copy_(torch.from_numpy(x))
+            v = plan * u
+            torch.cuda.synchronize()
+            got = v.logical().cpu().numpy()
+            seen.update(plan.kernels(0) + plan.kernels(2))
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert err <= tol, (dims, name, k, err, plan.kernels(0))
+            if ref is None:
+                ref = got
+            else:
+                d = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+                assert d <= 1e-6, (dims, name, k, d, plan.kernels(0))
+    chunked = sorted(s for s in seen if ",S=" in s and any(",%d," % m in s for m in (192, 320, 384)))
+    print(chunked)
+    for m in (",320,", ",384,"):
+        assert any(m + "c2c" in s for s in chunked), (m, chunked)
+    assert any("r2c" in s for s in chunked), chunked
