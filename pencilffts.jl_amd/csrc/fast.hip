@@ -1765,6 +1765,16 @@ static void add_mixed_chunks(std::vector<Entry>& v) {
   add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
 }
 
+// Row-chunked transposed stores for the DCT-II / DCT-III of complex data
+// (the mixed workload's third dimension)
+template <typename R, int M>
+static void add_dct_chunks(std::vector<Entry>& v) {
+  constexpr int TT = sizeof(R) == 4 ? 16 : 8;
+  add<R, M, TT, 1, K_DCT2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<R, M, TT, 1, K_DCT3C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<R, M, TT, 2, K_DCT3C, IO_LINE, IO_TILE, 0, 2>(v);
+}
+
 static const std::vector<Entry>& registry() {
   static std::vector<Entry> v = [] {
     std::vector<Entry> r;
@@ -1813,6 +1823,8 @@ static const std::vector<Entry>& registry() {
     add_mixed_chunks<192>(r);
     add_mixed_chunks<320>(r);
     add_mixed_chunks<384>(r);
+    add_dct_chunks<float, 256>(r);
+    add_dct_chunks<float, 512>(r);
     return r;
   }();
   return v;

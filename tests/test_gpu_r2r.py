@@ -88,3 +88,23 @@ def test_r2r_every_candidate(pf, gpu, kind, data, real_type):
     print(kind, data, real_type, sorted(seen))
     tag = "r," if data == "real" else "c,"
     assert any(tag in s and ("dct" in s or "dst" in s or "dht" in s) for s in seen), seen
+
+
+@pytest.mark.parametrize("kind", ["REDFT10", "REDFT01"])
+def test_dct_row_chunks(pf, gpu, kind):
+    """The row-chunked transposed stores of the 256-point DCT-II / DCT-III
+    kernels of complex data (`S=2`), forced among the candidates, against the
+    oracle."""
+    T = pf.Transforms
+    L = pf._lib
+    dims = (32, 256, 64)   # FFT along 32, the DCT of complex data along 256 and 64
+    tol = 5e-7 * np.log2(int(np.prod(dims)))
+    seen = set()
+    for k in range(24):
+        err_f, err_b, meta = run_case(pf, dims, _trs(T, kind, "complex"), 0, seed=k,
+                                      flags=L.FLAG_CANDIDATE(k))
+        seen.update(meta.kernels)
+        assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
+    chunked = [s for s in seen if ",256,dct" in s and ",S=2" in s]
+    print(sorted(chunked))
+    assert chunked, sorted(seen)
