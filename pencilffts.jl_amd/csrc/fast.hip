@@ -1206,33 +1206,50 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       const int t = q * TQ + g;
       const bool active = t < c.Tact;
       if constexpr (GATHER) {
-        // element x_k of this line, from the tile or from the line's LDS slot
-        if constexpr (LD == IO_LINE) {
+        // element x_k of this line: from the tile; with line loads, from the
+        // line's LDS slot (filled first), or -- for the gathers of x_k with
+        // x_{M-k} (c2r merge, DCT-III / DST-III pre-twiddles) -- straight
+        // from global memory where the gather needs it: each wavefront load
+        // covers one contiguous run, ascending or descending, and the second
+        // touch of an element comes from the caches.  (The Makhoul reorder's
+        // stride-2 reads measured slower that way: those lines go through
+        // the slot.)
+        constexpr bool DIRECT =
+            LD == IO_LINE && (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
+        [[maybe_unused]] const C<R>* lin =
+            (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+        [[maybe_unused]] const R* lia = (const R*)a.in + c.in_base + (int64_t)t * a.lsin[0];
+        if constexpr (LD == IO_LINE && !DIRECT) {
           if (q > 0) line_sync<L>();
-          if constexpr (RP) {
-            // the pair's two real lines, lsin[0] / 2 apart
-            const R* ia = (const R*)a.in + c.in_base + (int64_t)t * a.lsin[0];
-            const R* ib = ia + a.lsin[0] / 2;
-            if (active)
-              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = mk(ia[k], ib[k]);
-          } else {
-            const C<R>* in = (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
-            if (active)
-              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = in[k];
+          if (active) {
+            // real pairs: the pair's two real lines, lsin[0] / 2 apart
+            if constexpr (RP)
+              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = mk(lia[k], lia[a.lsin[0] / 2 + k]);
+            else
+              for (int k = l; k < ROWS_IN; k += L) line[pad8(k)] = lin[k];
           }
           line_sync<L>();
         }
         auto X = [&](int k) -> C<R> {
-          if constexpr (LD == IO_TILE) return tile[tile_idx<R, T, SW>(k, t)];
-          else return line[pad8(k)];
+          if constexpr (LD == IO_TILE) {
+            return tile[tile_idx<R, T, SW>(k, t)];
+          } else if constexpr (!DIRECT) {
+            return line[pad8(k)];
+          } else {
+            if (!active) return mk(R(0), R(0));
+            if constexpr (RP) return mk(lia[k], lia[a.lsin[0] / 2 + k]);
+            else return lin[k];
+          }
         };
         auto XK = [&](int p) -> C<R> {    // x_k, k = l + L p
           if constexpr (LD == IO_TILE) return TK(p, t, 0);
-          else return LK(p);
+          else if constexpr (!DIRECT) return LK(p);
+          else return X(l + L * p);
         };
         auto XMK = [&](int p) -> C<R> {   // x_{M-k} for k > 0 (x_M when k = 0)
           if constexpr (LD == IO_TILE) return TMK(p, t);
-          else return LMK(p);
+          else if constexpr (!DIRECT) return LMK(p);
+          else return X(M - l - L * p);
         };
 #pragma unroll
         for (int p = 0; p < P; ++p) {
