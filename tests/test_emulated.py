@@ -222,3 +222,21 @@ def test_four_step_with_bluestein(pf, emu):
         assert err_f <= tol and err_b <= tol, (rt, err_f, err_b)
         _, full = meta_for(T, dims, (T.FFT(),), CODE_NAMES[rt], True, ())
         assert emu.bluestein(dims, [t.code for t in full], rt) == [0, 4096, 0]
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims,trs_name,permute,extra", [
+    ((16384, 2), "FFT", False, (3,)),     # four-step, unpermuted, extra dims
+    ((3, 16384), "BFFT", True, (2,)),
+    ((1031, 4), "FFT", False, (2,)),      # Bluestein, unpermuted, extra dims
+    ((6, 134, 4), "RFFT", False, ()),
+])
+def test_long_and_prime_lines_layouts(pf, emu, dims, trs_name, permute, extra, real_type):
+    """The four-step split and the Bluestein form under the layout options
+    of transforms.jl:414-462 (permute_dims=false) and extra dims
+    (plans.jl:164-182)."""
+    T = pf.Transforms
+    err_f, err_b = emu_case(pf, emu, dims, (getattr(T, trs_name)(),), real_type,
+                            permute=permute, extra=extra)
+    tol = tol_for(CODE_NAMES[real_type], tuple(dims) + tuple(extra))
+    assert err_f <= tol and err_b <= tol, (err_f, err_b, tol)

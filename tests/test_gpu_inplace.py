@@ -61,6 +61,8 @@ CASES = [
     # the specialised kernels, in place and transposing between U and the scratch
     ((256, 128, 64), "FFT!", 0), ((512, 64, 32), "RFFT!", 0), ((256, 64, 32), "RFFT!", 1),
     ((1024, 32, 16), "FFT!", 0), ((384, 96, 64), "RFFT!", 0),
+    # four-step split (16384 = 128 x 128) and Bluestein (1031) lines, in place
+    ((16384, 32), "FFT!", 0), ((16384, 16), "FFT!", 1), ((1031, 24, 8), "FFT!", 1),
 ]
 
 

@@ -151,3 +151,15 @@ def test_inplace_requires_one_buffer(pf, emu):
         assert "must be the same" in emu.lib.pfft_emu_last_error().decode()
     finally:
         emu.lib.pfft_emu_plan_destroy(h)
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims", [(16384, 3), (4, 12288), (1031, 6)])
+def test_inplace_long_and_prime_lines(pf, emu, dims, real_type):
+    """In-place FFT! plans whose lines take the four-step split (16384,
+    12288) or the Bluestein form (1031): the split's intermediate moves
+    through the scratch buffer like any transposing pass."""
+    T = pf.Transforms
+    err_f, err_b, _ = inplace_case(pf, emu, dims, (T.FFT_(),), real_type, seed=5)
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    assert err_f <= tol and err_b <= tol, (err_f, err_b)
