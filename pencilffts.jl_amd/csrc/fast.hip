@@ -652,14 +652,24 @@ static __constant__ float kCos32[32] = {
     0.98078528040323044913f};
 
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-// hi + lo split of 8 floats into two FP16 operand slices
+// hi + lo split of 8 floats into two FP16 operand slices, both rounded to
+// nearest: pairwise, so that one packed conversion (v_cvt_pk_f16_f32) rounds
+// two values and the hi halves convert back from that packed result -- 3
+// VALU per value instead of 4 (a separate scalar conversion for the
+// back-conversion), with the same bits
 __device__ __forceinline__ void split8(const float (&x)[8], f16x8& hi, f16x8& lo) {
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const _Float16 h = (_Float16)x[t];
-    hi[t] = h;
-    lo[t] = (_Float16)(x[t] - (float)h);
+  for (int t = 0; t < 8; t += 2) {
+    const f32x2 x2 = {x[t], x[t + 1]};
+    const f16x2 h = __builtin_convertvector(x2, f16x2);
+    const f32x2 hb = __builtin_convertvector(h, f32x2);
+    const f16x2 l2 = __builtin_convertvector(x2 - hb, f16x2);
+    hi[t] = h[0];
+    hi[t + 1] = h[1];
+    lo[t] = l2[0];
+    lo[t + 1] = l2[1];
   }
 }
 __device__ __forceinline__ f16x8 neg8(f16x8 v) {
