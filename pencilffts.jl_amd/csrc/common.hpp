@@ -37,7 +37,15 @@ enum LineKind : int32_t {
   L_DST2 = 11,     // RODFT10: DCT2 of (-1)^j x, reversed output
   L_DST3 = 12,     // RODFT01: (-1)^k DCT3 of the reversed input
   L_DST4 = 13,     // RODFT11: (-1)^k DCT4 of the reversed input
-  L_DHT = 14       // DHT: Re F - Im F
+  L_DHT = 14,      // DHT: Re F - Im F
+  // REDFT11 / RODFT11 of even length n: the n/2-point FFTs of two real lines
+  // a, b (the halves z_m = (x_2m + i x_{n-1-2m}) c_m) interleaved into one
+  // n-point FFT W, split afterwards as Z_m = (W_m + W_{m+n/2}) / 2 and
+  // w^m (W_m - W_{m+n/2}) / 2.  The generic pass feeds one real line per work
+  // line (b = 0); the fast pass two (the parts of complex data, or a pair of
+  // real lines).
+  L_DCT4H = 15,
+  L_DST4H = 16
 };
 
 // Descriptor of one pass over the whole array (passed by value to kernels).

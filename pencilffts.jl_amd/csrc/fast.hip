@@ -819,6 +819,14 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 //   K_DHTC  DHT of complex data: one M-point FFT of x = a + i b, then
 //           A = (Z_k + conj Z_{M-k}) / 2, B = (Z_k - conj Z_{M-k}) / 2i and
 //           y_k = (Re A - Im A) + i (Re B - Im B)
+//   K_DCT4C REDFT11 of complex data: the M/2-point FFTs of the halves
+//           z_m = (x_2m + i x_{M-1-2m}) c_m of the real part (even points)
+//           and of the imaginary part (odd points) interleaved into one
+//           M-point FFT W; then Z_m = (W_m + W_{m+M/2}) / 2 for the real part,
+//           w_M^-m (W_m - W_{m+M/2}) / 2 for the imaginary part, and
+//           y_2m = 2 Re(d_m Z_m), y_{M-1-2m} = -2 Im(d_m Z_m), d_m = e^{-i pi m / M}
+//   K_DST4C RODFT11 of complex data: y_k = (-1)^k DCT-IV(x_{M-1-j})_k, the
+//           reversal folded into the halves' gather, the sign into the output
 // K_REAL_PAIRS (OR-ed into the R2R kinds): the same transforms of real data,
 // two real lines (2j, 2j + 1) forming the real and imaginary parts of one
 // complex line.  On the side where the pass is transposed the two lines are
@@ -826,7 +834,7 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 // side the pair's real and imaginary parts are read from (written to) two
 // lines, a.lsin[0] / 2 (a.lsout[0] / 2) elements apart.
 enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4, K_DST2C = 5, K_DST3C = 6,
-       K_DHTC = 7 };
+       K_DHTC = 7, K_DCT4C = 8, K_DST4C = 9 };
 constexpr int K_REAL_PAIRS = 16;
 enum { IO_LINE = 0, IO_TILE = 1 };
 
@@ -1079,7 +1087,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
   constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
   constexpr bool DCTLIKE = KIND == K_DCT2C || KIND == K_DCT3C || KIND == K_DST2C ||
-                           KIND == K_DST3C;
+                           KIND == K_DST3C || KIND == K_DCT4C || KIND == K_DST4C;
   constexpr bool GATHER = KIND == K_C2R || DCTLIKE;                 // pre reads (k, M-k)
   constexpr bool POST = KIND == K_R2C || DCTLIKE || KIND == K_DHTC;  // post reads LDS
   constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
@@ -1280,6 +1288,13 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           } else if constexpr (KIND == K_DST2C) {
             // the reorder of (-1)^j x_j: the odd-indexed points change sign
             z = 2 * k < M ? X(2 * k) : scl(X(2 * M - 2 * k - 1), R(-1));
+          } else if constexpr (KIND == K_DCT4C || KIND == K_DST4C) {
+            // point 2m + s: the half z_m of part s (0 real, 1 imaginary);
+            // the DST reads the line reversed
+            const int m2 = k & ~1;
+            const C<R> x0 = X(m2), x1 = X(M - 1 - m2);
+            const C<R> xa = KIND == K_DCT4C ? x0 : x1, xb = KIND == K_DCT4C ? x1 : x0;
+            z = aux[m2 >> 1] * ((k & 1) ? mk(xa.im, xb.im) : mk(xa.re, xb.re));
           } else if constexpr (KIND == K_DST3C) {
             // the pre-twiddle of the reversed line x'_j = x_{M-1-j}
             const C<R> xk = X(M - 1 - k);
@@ -1420,6 +1435,15 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           } else if constexpr (KIND == K_DST3C) {
             // inverse Makhoul reorder, then the sign (-1)^k
             Y[p] = scl(line[pad8((k & 1) ? M - 1 - (k >> 1) : (k >> 1))], (k & 1) ? -sc : sc);
+          } else if constexpr (KIND == K_DCT4C || KIND == K_DST4C) {
+            // output k = 2m (k even) or M - 1 - 2m (k odd) of each part, from
+            // W_m and W_{m+M/2}; e_m = aux[M/2 + m] = e^{i pi m / M}
+            const int m = (k & 1) ? M / 2 - 1 - (k >> 1) : (k >> 1);
+            const C<R> Wm = line[pad8(m)], Wh = line[pad8(m + M / 2)];
+            const C<R> e = aux[M / 2 + m];
+            const C<R> ya = conj(e) * (Wm + Wh), yb = e * (Wm - Wh);
+            const R s = (k & 1) ? (KIND == K_DST4C ? sc : -sc) : sc;
+            Y[p] = scl((k & 1) ? mk(ya.im, yb.im) : mk(ya.re, yb.re), s);
           } else if constexpr (KIND == K_DHTC) {
             // the transforms A, B of the real and imaginary parts, then
             // Re - Im of each
@@ -1665,6 +1689,10 @@ static void add_dct(std::vector<Entry>& v) {
   add_r2r_kind<R, M, K_DST2C | K_REAL_PAIRS, false>(v);
   add_r2r_kind<R, M, K_DST3C | K_REAL_PAIRS, false>(v);
   add_r2r_kind<R, M, K_DHTC | K_REAL_PAIRS, false>(v);
+  add_r2r_kind<R, M, K_DCT4C, false>(v);
+  add_r2r_kind<R, M, K_DST4C, false>(v);
+  add_r2r_kind<R, M, K_DCT4C | K_REAL_PAIRS, false>(v);
+  add_r2r_kind<R, M, K_DST4C | K_REAL_PAIRS, false>(v);
 }
 
 template <typename R, int M>
@@ -1957,6 +1985,8 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
     case L_DST2: kind = K_DST2C; break;
     case L_DST3: kind = K_DST3C; break;
     case L_DHT: kind = K_DHTC; break;
+    case L_DCT4H: kind = K_DCT4C; break;
+    case L_DST4H: kind = K_DST4C; break;
     default: return out;
   }
   // R2R of real data: pairs of lines as complex lines
@@ -2086,7 +2116,8 @@ const char* fast_name(int variant) {
   if (variant < 1 || idx < 0 || idx >= (int)reg.size()) return "generic";
   const Entry& e = reg[idx];
   // R2R kinds end in c (complex data) or r (real data, line pairs)
-  static const char* kinds[8] = {"c2c", "r2c", "c2r", "dct2", "dct3", "dst2", "dst3", "dht"};
+  static const char* kinds[10] = {"c2c", "r2c", "c2r",  "dct2", "dct3",
+                                  "dst2", "dst3", "dht", "dct4", "dst4"};
   const int kb = e.kind & 15;
   const std::string kname =
       std::string(kinds[kb]) + (kb >= K_DCT2C ? ((e.kind & K_REAL_PAIRS) ? "r" : "c") : "");

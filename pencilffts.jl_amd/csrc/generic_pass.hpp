@@ -290,6 +290,18 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         v = (int64_t)j < n ? scl(aux[j], raw_real(a, raw, w, n - 1 - j))
                            : mk(zero, zero);
         break;
+      case L_DCT4H:
+      case L_DST4H:
+        // even slots: the half z_m = (x_2m + i x_{n-1-2m}) c_m (reversed
+        // line for the DST); odd slots: the second real line, none here
+        if (j & 1) {
+          v = mk(zero, zero);
+        } else {
+          const int64_t m2 = j;   // 2m
+          const R xa = raw_real(a, raw, w, m2), xb = raw_real(a, raw, w, n - 1 - m2);
+          v = aux[j >> 1] * (a.kind == L_DCT4H ? mk(xa, xb) : mk(xb, xa));
+        }
+        break;
       case L_DST1: {
         const int64_t jj = j;
         if (jj == 0 || jj == n + 1) v = mk(zero, zero);
@@ -564,6 +576,15 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
       const cpx<R> z = aux[n + k] * L[wp((uint32_t)(k))];
       const R y = R(2) * z.re;
       return (a.kind == L_DST4 && (k & 1)) ? -y : y;
+    }
+    case L_DCT4H:
+    case L_DST4H: {
+      // y_2m = Re(conj(e_m) (W_m + W_{m+n/2})), y_{n-1-2m} = -Im(...)
+      const int64_t h = n / 2;
+      const int64_t m = (k & 1) ? h - 1 - (k >> 1) : (k >> 1);
+      const cpx<R> z = conj(aux[h + m]) * (L[wp((uint32_t)m)] + L[wp((uint32_t)(m + h))]);
+      const R y = (k & 1) ? -z.im : z.re;
+      return (a.kind == L_DST4H && (k & 1)) ? -y : y;
     }
     case L_DST1: return -L[wp((uint32_t)(k + 1))].im;
     case L_DHT: return L[wp((uint32_t)(k))].re - L[wp((uint32_t)(k))].im;

@@ -427,6 +427,16 @@ static int line_fft(Plan& p, PassPlan& pp) {
     for (int64_t k = 0; k < n; ++k) expm(k, 4 * n, &aux[2 * k], &aux[2 * k + 1]);
     for (int64_t k = 0; k < n; ++k)
       expm(2 * k + 1, 8 * n, &aux[2 * (n + k)], &aux[2 * (n + k) + 1]);
+  } else if (lk == L_DCT4H || lk == L_DST4H) {
+    // entries m < n/2: the pre-twiddle c_m = exp(-i pi (4m + 1) / 4n) of the
+    // half z_m; entries n/2 + m: e_m = exp(i pi m / n), the post-twiddle
+    // conj(e_m) of Z_m and the split factor w_n^-m folded into it
+    aux.resize(2 * n);
+    const int64_t h = n / 2;
+    for (int64_t m = 0; m < h; ++m) {
+      expm(4 * m + 1, 8 * n, &aux[2 * m], &aux[2 * m + 1]);
+      expm(-m, 2 * n, &aux[2 * (h + m)], &aux[2 * (h + m) + 1]);
+    }
   } else if (lk == L_C2C && M == 1024 && p.real_type == PFFT_F32) {
     // middle twiddles of the 32 x 32 MFMA form of 1024-point lines, in the
     // order its lanes read them: entry 64 i + l is w1024^(k2 n1) with
@@ -673,11 +683,18 @@ static int build_pass(Plan& p, int dim, int32_t kind, int32_t elt_in, int32_t el
         lk = L_DCT1; M = 2 * (n - 1); break;
       case PFFT_REDFT10: lk = L_DCT2; M = n; break;
       case PFFT_REDFT01: lk = L_DCT3; M = n; fft_bwd = 1; break;
-      case PFFT_REDFT11: lk = L_DCT4; M = 2 * n; break;
+      // even lengths: one n-point FFT holds two real lines' n/2-point FFTs
+      case PFFT_REDFT11:
+        if (n % 2 == 0) { lk = L_DCT4H; M = n; }
+        else { lk = L_DCT4; M = 2 * n; }
+        break;
       case PFFT_RODFT00: lk = L_DST1; M = 2 * (n + 1); break;
       case PFFT_RODFT10: lk = L_DST2; M = n; break;
       case PFFT_RODFT01: lk = L_DST3; M = n; fft_bwd = 1; break;
-      case PFFT_RODFT11: lk = L_DST4; M = 2 * n; break;
+      case PFFT_RODFT11:
+        if (n % 2 == 0) { lk = L_DST4H; M = n; }
+        else { lk = L_DST4; M = 2 * n; }
+        break;
       case PFFT_DHT: lk = L_DHT; M = n; break;
       default: return fail(PFFT_ERR_ARGUMENT, "unsupported r2r kind");
     }

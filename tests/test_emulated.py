@@ -67,6 +67,27 @@ def test_dimensionality_and_sizes(pf, emu, dims, permute, real_type):
         assert err_f <= tol and err_b <= tol, (trs, err_f, err_b)
 
 
+R2R_KINDS = ["REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01", "RODFT10",
+             "RODFT11", "DHT"]
+
+
+@pytest.mark.parametrize("real_type", [1, 0], ids=["F64", "F32"])
+@pytest.mark.parametrize("dims", [(16, 12, 6), (15, 9, 7)], ids=["even", "odd"])
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("kind", R2R_KINDS)
+def test_r2r_kinds(pf, emu, kind, data, dims, real_type):
+    """The nine kinds of src/Transforms/r2r.jl:5-15 on the generic pass, real
+    data ((R2R(kind),) x 3) and complex data ((FFT, R2R(kind), R2R(kind))), at
+    even lengths (REDFT11 / RODFT11 as two half-length FFTs in one) and odd
+    ones (their zero-padded 2n-point form)."""
+    T = pf.Transforms
+    k = getattr(T, kind)
+    trs = (T.R2R(k),) if data == "real" else (T.FFT(), T.R2R(k), T.R2R(k))
+    err_f, err_b = emu_case(pf, emu, dims, trs, real_type, seed=5)
+    tol = tol_for(CODE_NAMES[real_type], dims)
+    assert err_f <= tol and err_b <= tol, (err_f, err_b, tol)
+
+
 def test_mixed_config3_small(pf, emu):
     """(RFFT, FFT, R2R(REDFT10)) -- the mixed workload's transform tuple -- at a
     small size, both precisions."""

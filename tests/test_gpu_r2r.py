@@ -2,9 +2,10 @@
 :86-93) on the GPU against the FP64 oracle, at the lengths the specialised
 passes cover: real data ((R2R(kind),) x 3: every pass an R2R of real lines)
 and complex data ((FFT, R2R(kind), R2R(kind))).  REDFT10 / REDFT01 /
-RODFT10 / RODFT01 / DHT run on the specialised kernels (pre/post-twiddled
-FFTs; real data two lines per complex FFT), the others on the generic pass;
-each case records which.  Every specialised candidate is also forced once."""
+RODFT10 / RODFT01 / REDFT11 / RODFT11 / DHT run on the specialised kernels
+(pre/post-twiddled FFTs; two real lines per complex FFT), REDFT00 / RODFT00
+on the generic pass; each case records which.  Every specialised candidate is
+also forced once."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,7 @@ torch = pytest.importorskip("torch")
 
 KINDS = ["REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01", "RODFT10",
          "RODFT11", "DHT"]
-FAST = {"REDFT01", "REDFT10", "RODFT01", "RODFT10", "DHT"}
+FAST = {"REDFT01", "REDFT10", "REDFT11", "RODFT01", "RODFT10", "RODFT11", "DHT"}
 
 
 def _trs(T, kind, data):
