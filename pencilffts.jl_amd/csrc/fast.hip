@@ -827,6 +827,12 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 //           y_2m = 2 Re(d_m Z_m), y_{M-1-2m} = -2 Im(d_m Z_m), d_m = e^{-i pi m / M}
 //   K_DST4C RODFT11 of complex data: y_k = (-1)^k DCT-IV(x_{M-1-j})_k, the
 //           reversal folded into the halves' gather, the sign into the output
+//   K_DCT1C REDFT00 of complex data of n = M/2 + 1 points: the M-point FFT of
+//           the even extension (x_j, j <= M/2; x_{M-j} above) is the DCT-I of
+//           the real part plus i times that of the imaginary part, rows 0..M/2
+//   K_DST1C RODFT00 of complex data of n = M/2 - 1 points: the M-point FFT W
+//           of the odd extension (0, x_0 .. x_{n-1}, 0, -x_{n-1} .. -x_0)
+//           gives y_k = i W_{k+1}, k < n
 // K_REAL_PAIRS (OR-ed into the R2R kinds): the same transforms of real data,
 // two real lines (2j, 2j + 1) forming the real and imaginary parts of one
 // complex line.  On the side where the pass is transposed the two lines are
@@ -834,9 +840,20 @@ PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
 // side the pair's real and imaginary parts are read from (written to) two
 // lines, a.lsin[0] / 2 (a.lsout[0] / 2) elements apart.
 enum { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_DCT2C = 3, K_DCT3C = 4, K_DST2C = 5, K_DST3C = 6,
-       K_DHTC = 7, K_DCT4C = 8, K_DST4C = 9 };
+       K_DHTC = 7, K_DCT4C = 8, K_DST4C = 9, K_DCT1C = 10, K_DST1C = 11 };
 constexpr int K_REAL_PAIRS = 16;
 enum { IO_LINE = 0, IO_TILE = 1 };
+
+// points per line read and written (the r2c output and c2r input carry
+// M + 1, the DCT-I and DST-I lines M/2 + 1 and M/2 - 1)
+template <int KIND, int M>
+constexpr int rows_in() {
+  return KIND == K_C2R ? M + 1 : KIND == K_DCT1C ? M / 2 + 1 : KIND == K_DST1C ? M / 2 - 1 : M;
+}
+template <int KIND, int M>
+constexpr int rows_out() {
+  return KIND == K_R2C ? M + 1 : KIND == K_DCT1C ? M / 2 + 1 : KIND == K_DST1C ? M / 2 - 1 : M;
+}
 
 // Output stores, plain or with the nontemporal (streaming) hint: the next
 // pass reads the output only after the whole pass, and whether keeping it in
@@ -1003,8 +1020,8 @@ __device__ __forceinline__ void tile_load(C<R>* tile, const PassArgs& a, int64_t
 template <typename R, int M, int T, int LPW, int KIND, int LD, int ST, int NSP, int MF = 0>
 constexpr size_t lds_base() {   // T / LPW line slots whatever the lanes per line
   constexpr size_t lines = MF == 3 ? 0 : (size_t)(T / LPW) * line_pitch<M>() * sizeof(C<R>);
-  constexpr int rows_ld = KIND == K_C2R ? M + 1 : M;
-  constexpr int rows_st = M / NSP + (KIND == K_R2C ? 1 : 0);
+  constexpr int rows_ld = rows_in<KIND, M>();
+  constexpr int rows_st = NSP == 1 ? rows_out<KIND, M>() : M / NSP + (KIND == K_R2C ? 1 : 0);
   constexpr size_t tld = LD == IO_TILE ? (size_t)rows_ld * T * sizeof(C<R>) : 0;
   constexpr size_t tst = ST == IO_TILE ? (size_t)rows_st * T * sizeof(C<R>) : 0;
   constexpr size_t tile = tld > tst ? tld : tst;
@@ -1084,11 +1101,16 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   static_assert(OPK == 0 || KIND == K_C2C, "spectral operators ride on c2c passes");
   static_assert(!RP || KIND >= K_DCT2C, "real pairs: R2R kinds");
   // inputs / outputs per line (the r2c output and c2r input carry M + 1)
-  constexpr int ROWS_IN = KIND == K_C2R ? M + 1 : M;
-  constexpr int ROWS_OUT = KIND == K_R2C ? M + 1 : M;
+  constexpr int ROWS_IN = rows_in<KIND, M>();
+  constexpr int ROWS_OUT = rows_out<KIND, M>();
+  // DCT-I / DST-I: an extended line of ROWS_IN points, rows OFF .. OFF +
+  // ROWS_OUT - 1 of its FFT kept
+  constexpr bool EXT = KIND == K_DCT1C || KIND == K_DST1C;
+  constexpr int OFF = KIND == K_DST1C ? 1 : 0;
+  static_assert(!EXT || NSP == 1, "extended lines: whole-tile stores");
   constexpr bool DCTLIKE = KIND == K_DCT2C || KIND == K_DCT3C || KIND == K_DST2C ||
                            KIND == K_DST3C || KIND == K_DCT4C || KIND == K_DST4C;
-  constexpr bool GATHER = KIND == K_C2R || DCTLIKE;                 // pre reads (k, M-k)
+  constexpr bool GATHER = KIND == K_C2R || DCTLIKE || EXT;          // pre reads (k, M-k)
   constexpr bool POST = KIND == K_R2C || DCTLIKE || KIND == K_DHTC;  // post reads LDS
   constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
   extern __shared__ __align__(16) char smem[];
@@ -1288,6 +1310,13 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           } else if constexpr (KIND == K_DST2C) {
             // the reorder of (-1)^j x_j: the odd-indexed points change sign
             z = 2 * k < M ? X(2 * k) : scl(X(2 * M - 2 * k - 1), R(-1));
+          } else if constexpr (KIND == K_DCT1C) {
+            // even extension
+            z = X(2 * k <= M ? k : M - k);
+          } else if constexpr (KIND == K_DST1C) {
+            // odd extension, zeros at 0 and M/2
+            z = 2 * k < M ? (k == 0 ? mk(R(0), R(0)) : X(k - 1))
+                          : (2 * k == M ? mk(R(0), R(0)) : scl(X(M - k - 1), R(-1)));
           } else if constexpr (KIND == K_DCT4C || KIND == K_DST4C) {
             // point 2m + s: the half z_m of part s (0 real, 1 imaginary);
             // the DST reads the line reversed
@@ -1507,6 +1536,10 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       } else {
 #pragma unroll
         for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], scq);
+        if constexpr (KIND == K_DST1C) {
+#pragma unroll
+          for (int p = 0; p < P; ++p) v[q][p] = mul_pi(v[q][p]);
+        }
         if constexpr (OPK) {
           if (a.op_side == 2) {
             const OpLine<R> ol = op_line<R>(a, otile, om0 + t);
@@ -1514,7 +1547,26 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
             for (int p = 0; p < P; ++p) v[q][p] = v[q][p] * op_factor<R>(a, ol, l + L * p);
           }
         }
-        if constexpr (ST == IO_LINE) {
+        if constexpr (EXT && ST == IO_LINE) {
+          // rows OFF .. OFF + ROWS_OUT - 1 of the extended line's transform
+          C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+          R* oa = (R*)a.out + c.out_base + (int64_t)t * a.lsout[0];
+          R* ob = oa + a.lsout[0] / 2;
+          if (active) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              const int k = l + L * p;
+              if (k >= OFF && k < OFF + ROWS_OUT) {
+                if constexpr (RP) {
+                  st_real<NT != 0>(&oa[k - OFF], v[q][p].re);
+                  st_real<NT != 0>(&ob[k - OFF], v[q][p].im);
+                } else {
+                  st_line<NT != 0>(&out[k - OFF], v[q][p]);
+                }
+              }
+            }
+          }
+        } else if constexpr (ST == IO_LINE) {
           C<R>* out = (C<R>*)a.out + c.out_base + (int64_t)t * a.lsout[0];
           if (active) {
             if (OPK && a.op_acc) {
@@ -1546,6 +1598,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         for (int q = 0; q < LPW; ++q) {
           if constexpr (JM && NSP == 1) break;   // written line by line above
           const int t = q * TQ + g;
+          if constexpr (EXT) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              const int k = l + L * p;
+              if (k >= OFF && k < OFF + ROWS_OUT) tile[tile_idx<R, T, SW>(k - OFF, t)] = v[q][p];
+            }
+            continue;
+          }
 #pragma unroll
           for (int p = h * PCH; p < (h + 1) * PCH; ++p) TK(p, t, -h * RCH) = v[q][p];
           if constexpr (KIND == K_R2C)
@@ -1693,6 +1753,26 @@ static void add_dct(std::vector<Entry>& v) {
   add_r2r_kind<R, M, K_DST4C, false>(v);
   add_r2r_kind<R, M, K_DCT4C | K_REAL_PAIRS, false>(v);
   add_r2r_kind<R, M, K_DST4C | K_REAL_PAIRS, false>(v);
+}
+
+// DCT-I / DST-I (complex data and real line pairs) as M-point FFTs of the
+// extended lines
+template <typename R, int M>
+static void add_ext(std::vector<Entry>& v) {
+  constexpr int TL = 256 / Cfg<M>::L;
+  constexpr int TT = sizeof(R) == 4 && M < 1024 ? 16 : 8;
+  add<R, M, TL, 1, K_DCT1C, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DCT1C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DCT1C, IO_TILE, IO_LINE>(v);
+  add<R, M, TL, 1, K_DST1C, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DST1C, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DST1C, IO_TILE, IO_LINE>(v);
+  add<R, M, TL, 1, K_DCT1C | K_REAL_PAIRS, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DCT1C | K_REAL_PAIRS, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DCT1C | K_REAL_PAIRS, IO_TILE, IO_LINE>(v);
+  add<R, M, TL, 1, K_DST1C | K_REAL_PAIRS, IO_LINE, IO_LINE>(v);
+  add<R, M, TT, 1, K_DST1C | K_REAL_PAIRS, IO_LINE, IO_TILE>(v);
+  add<R, M, TT, 1, K_DST1C | K_REAL_PAIRS, IO_TILE, IO_LINE>(v);
 }
 
 template <typename R, int M>
@@ -1880,6 +1960,13 @@ static const std::vector<Entry>& registry() {
     add_mixed_chunks<384>(r);
     add_dct_chunks<float, 256>(r);
     add_dct_chunks<float, 512>(r);
+    add_ext<float, 128>(r);
+    add_ext<float, 256>(r);
+    add_ext<float, 512>(r);
+    add_ext<float, 1024>(r);
+    add_ext<double, 128>(r);
+    add_ext<double, 256>(r);
+    add_ext<double, 512>(r);
     return r;
   }();
   return v;
@@ -1987,6 +2074,8 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
     case L_DHT: kind = K_DHTC; break;
     case L_DCT4H: kind = K_DCT4C; break;
     case L_DST4H: kind = K_DST4C; break;
+    case L_DCT1: kind = K_DCT1C; break;
+    case L_DST1: kind = K_DST1C; break;
     default: return out;
   }
   // R2R of real data: pairs of lines as complex lines
@@ -2116,8 +2205,8 @@ const char* fast_name(int variant) {
   if (variant < 1 || idx < 0 || idx >= (int)reg.size()) return "generic";
   const Entry& e = reg[idx];
   // R2R kinds end in c (complex data) or r (real data, line pairs)
-  static const char* kinds[10] = {"c2c", "r2c", "c2r",  "dct2", "dct3",
-                                  "dst2", "dst3", "dht", "dct4", "dst4"};
+  static const char* kinds[12] = {"c2c",  "r2c",  "c2r", "dct2", "dct3", "dst2",
+                                  "dst3", "dht", "dct4", "dst4", "dct1", "dst1"};
   const int kb = e.kind & 15;
   const std::string kname =
       std::string(kinds[kb]) + (kb >= K_DCT2C ? ((e.kind & K_REAL_PAIRS) ? "r" : "c") : "");

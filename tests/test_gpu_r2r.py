@@ -4,8 +4,10 @@ passes cover: real data ((R2R(kind),) x 3: every pass an R2R of real lines)
 and complex data ((FFT, R2R(kind), R2R(kind))).  REDFT10 / REDFT01 /
 RODFT10 / RODFT01 / REDFT11 / RODFT11 / DHT run on the specialised kernels
 (pre/post-twiddled FFTs; two real lines per complex FFT), REDFT00 / RODFT00
-on the generic pass; each case records which.  Every specialised candidate is
-also forced once."""
+on the generic pass at these lengths (extended lines of 510 / 514 points); at
+n = 2^k + 1 / 2^k - 1 they have specialised kernels too (the last tests).
+Each case records which kernels ran.  Every specialised candidate is also
+forced once."""
 import numpy as np
 import pytest
 
@@ -109,3 +111,62 @@ def test_dct_row_chunks(pf, gpu, kind):
     chunked = [s for s in seen if ",256,dct" in s and ",S=2" in s]
     print(sorted(chunked))
     assert chunked, sorted(seen)
+
+
+# DCT-I / DST-I lengths with a specialised kernel: n = M/2 + 1 (REDFT00) and
+# n = M/2 - 1 (RODFT00) for the extended lines' FFT length M
+EXT_LENGTHS = {0: {"REDFT00": [129, 257, 513], "RODFT00": [127, 255, 511]},
+               1: {"REDFT00": [129, 257], "RODFT00": [127, 255]}}
+EXT_CASES = [(kind, n, data, rt) for rt in (0, 1) for kind in ("REDFT00", "RODFT00")
+             for n in EXT_LENGTHS[rt][kind] for data in ("complex", "real")]
+
+
+def _ext_case(T, kind, n, data):
+    k = T.R2R(getattr(T, kind))
+    if data == "real":   # the last dim's real lines as pairs
+        return (64, 64, n), (T.R2R(T.REDFT10), T.R2R(T.REDFT10), k)
+    return (32, 32, n), (T.FFT(), T.FFT(), k)
+
+
+@pytest.mark.parametrize("kind,n,data,real_type", EXT_CASES,
+                         ids=["%s-%d-%s-%s" % (k, n, d, "F32" if r == 0 else "F64")
+                              for k, n, d, r in EXT_CASES])
+def test_dct1_dst1_specialised(pf, gpu, kind, n, data, real_type):
+    """REDFT00 / RODFT00 (src/Transforms/r2r.jl:5-15) along the last dim at
+    the lengths of the specialised kernels (the M-point FFT of the even / odd
+    extension, M a power of two), complex data and real line pairs, against
+    the oracle.  The pass has one transposed side (the load forward, the
+    store backward); both directions run the specialised kernel."""
+    T = pf.Transforms
+    dims, trs = _ext_case(T, kind, n, data)
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(int(np.prod(dims)))
+    err_f, err_b, meta = run_case(pf, dims, trs, real_type, seed=n)
+    tag = ",dct1" if kind == "REDFT00" else ",dst1"
+    ext = [s for s in meta.kernels if tag in s]
+    print(kind, n, data, real_type, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert len(ext) == 2, meta.kernels
+    assert any("tile->line" in s for s in ext) and any("line->tile" in s for s in ext), ext
+    assert err_f <= tol and err_b <= tol
+
+
+@pytest.mark.parametrize("real_type", [0, 1], ids=["F32", "F64"])
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("kind", ["REDFT00", "RODFT00"])
+def test_dct1_dst1_every_candidate(pf, gpu, kind, data, real_type):
+    """Every specialised DCT-I / DST-I candidate forced once (both tile
+    orders, reversed tiles, streaming stores)."""
+    T = pf.Transforms
+    L = pf._lib
+    n = 257 if kind == "REDFT00" else 255
+    dims, trs = _ext_case(T, kind, n, data)
+    tol = 1e-13 if real_type == 1 else 5e-7 * np.log2(int(np.prod(dims)))
+    seen = set()
+    for k in range(8):
+        rev = L.FLAG_REVERSE_TILES | L.FLAG_STREAM_STORES if k % 2 else 0
+        err_f, err_b, meta = run_case(pf, dims, trs, real_type, seed=k,
+                                      flags=L.FLAG_CANDIDATE(k) | rev)
+        seen.update(meta.kernels)
+        assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
+    tag = ",dct1" if kind == "REDFT00" else ",dst1"
+    print(kind, data, real_type, sorted(s for s in seen if tag in s))
+    assert any(tag in s for s in seen), seen
