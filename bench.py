@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--extra", action="append", default=None,
-                    choices=["plain", "small_64_f64", "precision_1024", "batched_graph"],
+                    choices=["plain", "small_64_f64", "precision_1024", "batched_graph", "r2r"],
                     help="run only the named extra workload(s)")
     ap.add_argument("--no-checks", action="store_true",
                     help="skip the Float64-plan error and the copy-ceiling measurement "
@@ -346,9 +346,11 @@ def pair_entry(torch, pf, plan, u, v, steps, warmup, label):
     ms = time_pairs(torch, pf, plan, u, v, steps, warmup)
     pb = pass_bytes(pf, plan)
     pair = 2 * sum(pb)
+    fwd, bwd = pass_times(pf, plan, u, v, 5)
     return {"workload": label, "ms_per_pair": round(ms, 4),
             "hbm_frac": round(pair / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "pair_algorithmic_bytes": pair,
+            "passes_ms": {"fwd": [round(t, 4) for t in fwd], "bwd": [round(t, 4) for t in bwd]},
             "kernels": plan.kernels(0) + plan.kernels(2)}
 
 
@@ -388,6 +390,29 @@ def extra_plain(torch, pf, steps, warmup):
             label += ", 32 Fourier-Chebyshev modes + 1e-3 noise"
         else:
             fill(torch, u, g)
+        res[key] = pair_entry(torch, pf, plan, u, v, steps, warmup, label)
+        del plan, u, v
+        torch.cuda.empty_cache()
+    return res
+
+
+def extra_r2r(torch, pf, steps, warmup):
+    """R2R workloads beyond the DCT-II pair: a channel-flow layout with a
+    Chebyshev (DCT-I, Gauss-Lobatto) wall-normal dimension, and the DCT-IV
+    (self-inverse) on every dimension."""
+    T = pf.Transforms
+    res = {}
+    g = torch.Generator(device="cuda").manual_seed(11)
+    for key, dims, tr, label in (
+            ("chan_512x512x513_f32", (512, 512, 513),
+             (T.RFFT(), T.FFT(), T.R2R(T.REDFT00)),
+             "512x512x513 Float32 (RFFT, FFT, REDFT00) pair"),
+            ("redft11_512_f32", (512,) * 3, T.R2R(T.REDFT11),
+             "512^3 Float32 REDFT11 x3 pair")):
+        plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, T.Float32)
+        u = pf.allocate_input(plan)
+        v = pf.allocate_output(plan)
+        fill(torch, u, g)
         res[key] = pair_entry(torch, pf, plan, u, v, steps, warmup, label)
         del plan, u, v
         torch.cuda.empty_cache()
@@ -527,7 +552,8 @@ def extra_batched(torch, pf, steps, warmup):
 def run_extras(torch, pf, steps, warmup, only=None):
     res = {}
     for key, fn in (("plain", extra_plain), ("small_64_f64", extra_small_f64),
-                    ("precision_1024", extra_precision), ("batched_graph", extra_batched)):
+                    ("precision_1024", extra_precision), ("batched_graph", extra_batched),
+                    ("r2r", extra_r2r)):
         if only and key not in only:
             continue
         try:

@@ -1111,7 +1111,12 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   constexpr bool DCTLIKE = KIND == K_DCT2C || KIND == K_DCT3C || KIND == K_DST2C ||
                            KIND == K_DST3C || KIND == K_DCT4C || KIND == K_DST4C;
   constexpr bool GATHER = KIND == K_C2R || DCTLIKE || EXT;          // pre reads (k, M-k)
-  constexpr bool POST = KIND == K_R2C || DCTLIKE || KIND == K_DHTC;  // post reads LDS
+  // DCT-III / DST-III into a transposed tile: the inverse Makhoul reorder is
+  // a row permutation, applied by the tile writes themselves (no pass of the
+  // line through its LDS slot)
+  constexpr bool ROWPERM = (KIND == K_DCT3C || KIND == K_DST3C) && ST == IO_TILE && NSP == 1;
+  constexpr bool POST =
+      (KIND == K_R2C || DCTLIKE || KIND == K_DHTC) && !ROWPERM;   // post reads LDS
   constexpr int NX = KIND == K_R2C ? P + 1 : P;   // outputs per lane per line
   extern __shared__ __align__(16) char smem[];
   C<R>* lds = (C<R>*)smem;
@@ -1169,6 +1174,7 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
   }
   const C<R>* __restrict__ tw = PF ? (const C<R>*)(smem + TBL) : (const C<R>*)a.tw;
   const C<R>* __restrict__ aux = PF ? (const C<R>*)(smem + TBL) + M : (const C<R>*)a.aux;
+  static_assert(!DCTLIKE || L % 4 == 0, "DCT-IV twiddles by lane base");
   // backward FFTs run as conj(FFT(conj x))
   const bool conj_in =
       KIND == K_C2C ? a.bwd != 0 : (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
@@ -1248,14 +1254,15 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
       if constexpr (GATHER) {
         // element x_k of this line: from the tile; with line loads, from the
         // line's LDS slot (filled first), or -- for the gathers of x_k with
-        // x_{M-k} (c2r merge, DCT-III / DST-III pre-twiddles) -- straight
+        // x_{M-k} (c2r merge, DCT-III / DST-III pre-twiddles, the DCT-I /
+        // DST-I extensions) -- straight
         // from global memory where the gather needs it: each wavefront load
         // covers one contiguous run, ascending or descending, and the second
         // touch of an element comes from the caches.  (The Makhoul reorder's
         // stride-2 reads measured slower that way: those lines go through
         // the slot.)
         constexpr bool DIRECT =
-            LD == IO_LINE && (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C);
+            LD == IO_LINE && (KIND == K_C2R || KIND == K_DCT3C || KIND == K_DST3C || EXT);
         [[maybe_unused]] const C<R>* lin =
             (const C<R>*)a.in + c.in_base + (int64_t)t * a.lsin[0];
         [[maybe_unused]] const R* lia = (const R*)a.in + c.in_base + (int64_t)t * a.lsin[0];
@@ -1320,10 +1327,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
           } else if constexpr (KIND == K_DCT4C || KIND == K_DST4C) {
             // point 2m + s: the half z_m of part s (0 real, 1 imaginary);
             // the DST reads the line reversed
+            // the DST reads the line reversed.  m = l/2 + (L/2) p, so
+            // c_m = c_{l/2} w_M^{(L/4) p}: one table entry per lane, the
+            // others wave-uniform (scalar loads)
             const int m2 = k & ~1;
             const C<R> x0 = X(m2), x1 = X(M - 1 - m2);
             const C<R> xa = KIND == K_DCT4C ? x0 : x1, xb = KIND == K_DCT4C ? x1 : x0;
-            z = aux[m2 >> 1] * ((k & 1) ? mk(xa.im, xb.im) : mk(xa.re, xb.re));
+            const C<R> cm = aux[l >> 1] * tw[(L / 4) * p];
+            z = cm * ((k & 1) ? mk(xa.im, xb.im) : mk(xa.re, xb.re));
           } else if constexpr (KIND == K_DST3C) {
             // the pre-twiddle of the reversed line x'_j = x_{M-1-j}
             const C<R> xk = X(M - 1 - k);
@@ -1466,10 +1477,14 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
             Y[p] = scl(line[pad8((k & 1) ? M - 1 - (k >> 1) : (k >> 1))], (k & 1) ? -sc : sc);
           } else if constexpr (KIND == K_DCT4C || KIND == K_DST4C) {
             // output k = 2m (k even) or M - 1 - 2m (k odd) of each part, from
-            // W_m and W_{m+M/2}; e_m = aux[M/2 + m] = e^{i pi m / M}
+            // W_m and W_{m+M/2}; e_m = aux[M/2 + m] = e^{i pi m / M}, formed
+            // as e_{m(l)} w_M^{-+(L/4) p} (m runs up by (L/2) p for even k,
+            // down for odd k)
             const int m = (k & 1) ? M / 2 - 1 - (k >> 1) : (k >> 1);
             const C<R> Wm = line[pad8(m)], Wh = line[pad8(m + M / 2)];
-            const C<R> e = aux[M / 2 + m];
+            const C<R> wq = tw[(L / 4) * p];
+            const C<R> e = aux[M / 2 + ((l & 1) ? M / 2 - 1 - (l >> 1) : (l >> 1))] *
+                           ((l & 1) ? wq : conj(wq));
             const C<R> ya = conj(e) * (Wm + Wh), yb = e * (Wm - Wh);
             const R s = (k & 1) ? (KIND == K_DST4C ? sc : -sc) : sc;
             Y[p] = scl((k & 1) ? mk(ya.im, yb.im) : mk(ya.re, yb.re), s);
@@ -1535,7 +1550,8 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
         }
       } else {
 #pragma unroll
-        for (int p = 0; p < P; ++p) v[q][p] = scl(conj_out ? conj(v[q][p]) : v[q][p], scq);
+        for (int p = 0; p < P; ++p)
+          v[q][p] = scl(conj_out || ROWPERM ? conj(v[q][p]) : v[q][p], scq);
         if constexpr (KIND == K_DST1C) {
 #pragma unroll
           for (int p = 0; p < P; ++p) v[q][p] = mul_pi(v[q][p]);
@@ -1603,6 +1619,17 @@ __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_fast(PassArgs a) 
             for (int p = 0; p < P; ++p) {
               const int k = l + L * p;
               if (k >= OFF && k < OFF + ROWS_OUT) tile[tile_idx<R, T, SW>(k - OFF, t)] = v[q][p];
+            }
+            continue;
+          }
+          if constexpr (ROWPERM) {
+            // point m to row 2m (m < M/2) or 2M - 1 - 2m; DST-III: odd rows negated
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              const int m = l + L * p;
+              const int row = 2 * m < M ? 2 * m : 2 * M - 1 - 2 * m;
+              tile[tile_idx<R, T, SW>(row, t)] =
+                  KIND == K_DST3C && (row & 1) ? scl(v[q][p], R(-1)) : v[q][p];
             }
             continue;
           }
@@ -1749,10 +1776,27 @@ static void add_dct(std::vector<Entry>& v) {
   add_r2r_kind<R, M, K_DST2C | K_REAL_PAIRS, false>(v);
   add_r2r_kind<R, M, K_DST3C | K_REAL_PAIRS, false>(v);
   add_r2r_kind<R, M, K_DHTC | K_REAL_PAIRS, false>(v);
-  add_r2r_kind<R, M, K_DCT4C, false>(v);
+  add_r2r_kind<R, M, K_DCT4C, true>(v);
   add_r2r_kind<R, M, K_DST4C, false>(v);
-  add_r2r_kind<R, M, K_DCT4C | K_REAL_PAIRS, false>(v);
+  add_r2r_kind<R, M, K_DCT4C | K_REAL_PAIRS, true>(v);
   add_r2r_kind<R, M, K_DST4C | K_REAL_PAIRS, false>(v);
+}
+
+// 512- and 1024-point extended lines: the cross-lane exchange, and for 1024
+// points 16-line tiles (128-byte runs) with two lines per wavefront
+template <int M, int K>
+static void add_ext_more(std::vector<Entry>& v) {
+  constexpr int TL = 256 / Cfg<M>::L;
+  constexpr int TT = M < 1024 ? 16 : 8;
+  add<float, M, TL, 1, K, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, TT, 1, K, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TT, 1, K, IO_TILE, IO_LINE, 4>(v);
+  if constexpr (M == 1024) {
+    add<float, M, 16, 2, K, IO_LINE, IO_TILE>(v);
+    add<float, M, 16, 2, K, IO_TILE, IO_LINE>(v);
+    add<float, M, 16, 2, K, IO_LINE, IO_TILE, 4>(v);
+    add<float, M, 16, 2, K, IO_TILE, IO_LINE, 4>(v);
+  }
 }
 
 // DCT-I / DST-I (complex data and real line pairs) as M-point FFTs of the
@@ -1773,6 +1817,12 @@ static void add_ext(std::vector<Entry>& v) {
   add<R, M, TL, 1, K_DST1C | K_REAL_PAIRS, IO_LINE, IO_LINE>(v);
   add<R, M, TT, 1, K_DST1C | K_REAL_PAIRS, IO_LINE, IO_TILE>(v);
   add<R, M, TT, 1, K_DST1C | K_REAL_PAIRS, IO_TILE, IO_LINE>(v);
+  if constexpr (sizeof(R) == 4 && M >= 512) {
+    add_ext_more<M, K_DCT1C>(v);
+    add_ext_more<M, K_DST1C>(v);
+    add_ext_more<M, K_DCT1C | K_REAL_PAIRS>(v);
+    add_ext_more<M, K_DST1C | K_REAL_PAIRS>(v);
+  }
 }
 
 template <typename R, int M>
@@ -1900,6 +1950,30 @@ static void add_mixed_chunks(std::vector<Entry>& v) {
   add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
 }
 
+// 256- and 512-point DCT-II / III / IV lines with the cross-lane exchange
+template <int M>
+static void add_dct_xlane(std::vector<Entry>& v) {
+  constexpr int TL = 256 / Cfg<M>::L;
+  add<float, M, TL, 1, K_DCT2C, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT2C, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TL, 1, K_DCT3C, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT3C, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT3C, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TL, 1, K_DCT2C | K_REAL_PAIRS, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT2C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT2C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TL, 1, K_DCT3C | K_REAL_PAIRS, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT3C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT3C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TL, 1, K_DCT4C | K_REAL_PAIRS, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT4C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT4C | K_REAL_PAIRS, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, TL, 1, K_DCT4C, IO_LINE, IO_LINE, 4>(v);
+  add<float, M, 16, 1, K_DCT4C, IO_LINE, IO_TILE, 4>(v);
+  add<float, M, 16, 2, K_DCT4C, IO_LINE, IO_TILE, 4>(v);
+}
+
 // Row-chunked transposed stores for the DCT-II / DCT-III of complex data
 // (the mixed workload's third dimension)
 template <typename R, int M>
@@ -1967,6 +2041,8 @@ static const std::vector<Entry>& registry() {
     add_ext<double, 128>(r);
     add_ext<double, 256>(r);
     add_ext<double, 512>(r);
+    add_dct_xlane<256>(r);
+    add_dct_xlane<512>(r);
     return r;
   }();
   return v;

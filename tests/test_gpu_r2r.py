@@ -170,3 +170,23 @@ def test_dct1_dst1_every_candidate(pf, gpu, kind, data, real_type):
     tag = ",dct1" if kind == "REDFT00" else ",dst1"
     print(kind, data, real_type, sorted(s for s in seen if tag in s))
     assert any(tag in s for s in seen), seen
+
+
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("kind", ["REDFT10", "REDFT01", "REDFT11"])
+def test_dct_xlane_candidates(pf, gpu, kind, data):
+    """The 256- and 512-point DCT-II / III / IV kernels with the cross-lane
+    last exchange, forced among the candidates, against the oracle."""
+    T = pf.Transforms
+    L = pf._lib
+    dims = (512, 256, 32) if data == "real" else (32, 512, 256)
+    tol = 5e-7 * np.log2(int(np.prod(dims)))
+    seen = set()
+    for k in range(24):
+        err_f, err_b, meta = run_case(pf, dims, _trs(T, kind, data), 0, seed=k,
+                                      flags=L.FLAG_CANDIDATE(k))
+        seen.update(meta.kernels)
+        assert err_f <= tol and err_b <= tol, (k, meta.kernels, err_f, err_b)
+    xl = sorted(s for s in seen if ",xlane" in s and ("dct" in s))
+    print(kind, data, xl)
+    assert xl, sorted(seen)
