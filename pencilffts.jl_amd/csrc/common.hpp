@@ -12,7 +12,8 @@ namespace pfft {
 
 constexpr int kMaxDims = 8;       // transformed dims + the flattened extra dim
 constexpr int kMaxStages = 24;    // radix stages of one line FFT
-constexpr int kBlockThreads = 256;
+constexpr int kBlockThreads = 256;       // generic pass workgroup
+constexpr int kBlockThreadsHeavy = 512;  // ... for lines with direct prime stages or Bluestein
 constexpr int64_t kMaxLds = 160 * 1024;  // LDS bytes per workgroup (gfx950)
 // Line FFTs whose length has a prime factor above this run in the Bluestein
 // form (direct stages would cost that factor per output point).

@@ -20,8 +20,9 @@
 namespace pfft {
 
 // ---- generic LDS pass ---------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(kBlockThreads) void k_generic(PassArgs a) {
+// (NT: the workgroup size it is compiled for; the launch uses exactly that)
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_generic(PassArgs a) {
   extern __shared__ __align__(16) char lds[];
   for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     TileCtx c;
@@ -37,6 +38,32 @@ __global__ __launch_bounds__(kBlockThreads) void k_generic(PassArgs a) {
     }
     phase_post<R>(a, c, threadIdx.x, blockDim.x, lds);
     __syncthreads();
+  }
+}
+
+// the generic kernel instance of a pass (element type, workgroup size)
+static const void* generic_fn(int real_type, int threads) {
+  if (real_type == PFFT_F32)
+    return threads == kBlockThreadsHeavy ? (const void*)k_generic<float, kBlockThreadsHeavy>
+                                         : (const void*)k_generic<float, kBlockThreads>;
+  return threads == kBlockThreadsHeavy ? (const void*)k_generic<double, kBlockThreadsHeavy>
+                                       : (const void*)k_generic<double, kBlockThreads>;
+}
+
+static void launch_generic(const PassPlan& pp, int real_type, const PassArgs& a,
+                           hipStream_t st) {
+  const dim3 g((unsigned)pp.grid), b(pp.threads);
+  const size_t lds = (size_t)pp.lds_bytes;
+  if (real_type == PFFT_F32) {
+    if (pp.threads == kBlockThreadsHeavy)
+      hipLaunchKernelGGL((k_generic<float, kBlockThreadsHeavy>), g, b, lds, st, a);
+    else
+      hipLaunchKernelGGL((k_generic<float, kBlockThreads>), g, b, lds, st, a);
+  } else {
+    if (pp.threads == kBlockThreadsHeavy)
+      hipLaunchKernelGGL((k_generic<double, kBlockThreadsHeavy>), g, b, lds, st, a);
+    else
+      hipLaunchKernelGGL((k_generic<double, kBlockThreads>), g, b, lds, st, a);
   }
 }
 
@@ -181,12 +208,7 @@ static int launch_pass(const Plan& p, const PassPlan& pp, const void* src, void*
     a.scale = 1.0 / p.scale;
   }
   if (pp.variant == 0) {
-    if (p.real_type == PFFT_F32)
-      hipLaunchKernelGGL(k_generic<float>, dim3((unsigned)pp.grid), dim3(pp.threads),
-                         (size_t)pp.lds_bytes, st, a);
-    else
-      hipLaunchKernelGGL(k_generic<double>, dim3((unsigned)pp.grid), dim3(pp.threads),
-                         (size_t)pp.lds_bytes, st, a);
+    launch_generic(pp, p.real_type, a, st);
     return PFFT_OK;
   }
   return fast_launch(pp, a, st);
@@ -194,8 +216,7 @@ static int launch_pass(const Plan& p, const PassPlan& pp, const void* src, void*
 
 static int prepare_pass(const Plan& p, const PassPlan& pp) {
   if (pp.variant != 0) return fast_prepare(pp);
-  return p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
-                                 : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);
+  return set_lds_limit(generic_fn(p.real_type, pp.threads), pp.lds_bytes);
 }
 
 // Second tuning phase: every plain pass's leading candidates again, judged by
@@ -702,8 +723,7 @@ int device_prepare(Plan& p) {
   for (auto* list : {&p.fwd, &p.bwd}) {
     for (PassPlan& pp : *list) {
       if (pp.variant == 0) {
-        r = p.real_type == PFFT_F32 ? set_lds_limit((const void*)k_generic<float>, pp.lds_bytes)
-                                    : set_lds_limit((const void*)k_generic<double>, pp.lds_bytes);
+        r = set_lds_limit(generic_fn(p.real_type, pp.threads), pp.lds_bytes);
       } else {
         r = fast_prepare(pp);
       }
@@ -1052,12 +1072,7 @@ int device_execute(Plan& p, int dir, const void* in, void* out, void* work, size
     }
     a.adjoint = adjoint ? 1 : 0;
     if (pp.variant == 0) {
-      if (p.real_type == PFFT_F32)
-        hipLaunchKernelGGL(k_generic<float>, dim3((unsigned)pp.grid), dim3(pp.threads),
-                           (size_t)pp.lds_bytes, stream, a);
-      else
-        hipLaunchKernelGGL(k_generic<double>, dim3((unsigned)pp.grid), dim3(pp.threads),
-                           (size_t)pp.lds_bytes, stream, a);
+      launch_generic(pp, p.real_type, a, stream);
     } else {
       r = fast_launch(pp, a, stream);
       if (r) return r;
