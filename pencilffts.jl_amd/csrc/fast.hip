@@ -800,6 +800,8 @@ PFFT_CFG(160, 8, 20, 4, 4, 10)
 PFFT_CFG(192, 16, 12, 4, 4, 12)
 PFFT_CFG(320, 16, 20, 4, 4, 20)
 PFFT_CFG(384, 32, 12, 4, 4, 4, 6)
+PFFT_CFG(640, 32, 20, 4, 4, 4, 10)
+PFFT_CFG(768, 64, 12, 4, 4, 4, 12)
 #undef PFFT_CFG
 
 // kinds of the fast pass:
@@ -2043,6 +2045,10 @@ static const std::vector<Entry>& registry() {
     add_ext<double, 512>(r);
     add_dct_xlane<256>(r);
     add_dct_xlane<512>(r);
+    add_size<float, 640>(r);
+    add_size<float, 768>(r);
+    add_mixed_chunks<640>(r);
+    add_mixed_chunks<768>(r);
     return r;
   }();
   return v;

@@ -49,15 +49,18 @@ def test_fast_kernels_selected(pf, gpu):
     assert all(k.startswith("k_fast") for k in fwd + bwd), (fwd, bwd)
 
 
-MIXED_SHAPES = [(384, 96, 64), (192, 384, 32), (320, 160, 64), (96, 192, 48), (64, 320, 96)]
+MIXED_SHAPES = [(384, 96, 64), (192, 384, 32), (320, 160, 64), (96, 192, 48), (64, 320, 96),
+                (768, 64, 32), (1536, 32, 16), (640, 64, 32), (1280, 32, 16), (32, 768, 48),
+                (48, 640, 32)]
 
 
 @pytest.mark.parametrize("permute", [True, False], ids=["perm", "noperm"])
 @pytest.mark.parametrize("tname", ["RFFT", "FFT", "MIXED"])
 @pytest.mark.parametrize("dims", MIXED_SHAPES, ids=lambda d: "x".join(map(str, d)))
 def test_fast_mixed_radix(pf, gpu, dims, tname, permute):
-    """Lengths 3*2^k and 5*2^k (the mixed and batched workloads' sizes) and the
-    complex-data REDFT10/REDFT01 passes, Float32."""
+    """Lengths 3*2^k and 5*2^k (the mixed and batched workloads' sizes, and the
+    640- and 768-point lines, also as the half-length FFTs of 1280 and 1536
+    reals) and the complex-data REDFT10/REDFT01 passes, Float32."""
     T = pf.Transforms
     if tname == "MIXED":
         trs = (T.RFFT(), T.FFT(), T.R2R(T.REDFT10))
@@ -65,8 +68,11 @@ def test_fast_mixed_radix(pf, gpu, dims, tname, permute):
         trs = (getattr(T, tname)(),)
     err_f, err_b, meta = run_case(pf, dims, trs, 0, permute=permute, seed=5)
     tol = 5e-7 * np.log2(float(np.prod(dims)))
-    print(dims, tname, permute, "fwd %.2e rt %.2e" % (err_f, err_b))
+    print(dims, tname, permute, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
     assert err_f <= tol and err_b <= tol
+    if tname == "FFT" and dims[0] in (640, 768):
+        # the long dim's passes (forward and backward) run a specialised kernel
+        assert sum(",%d,c2c" % dims[0] in k for k in meta.kernels) == 2, meta.kernels
 
 
 def test_mixed_workload_full_size(pf, gpu):

@@ -49,12 +49,16 @@ T = pf.Transforms
 torch.zeros(1, device="cuda"); torch.cuda.synchronize()
 pf.import_wisdom(%(path)r)
 # the process's first use of the library's kernels loads its code objects
-# onto the device (once per process): a small plan that runs specialised
-# kernels, executed once, takes that cost here
+# onto the device (once per process; the kernels with streaming stores are a
+# second object): small plans that run specialised kernels of both, executed
+# once, take that cost here
 t0 = time.perf_counter()
 p64 = pf.PencilFFTPlan((64, 64, 64), T.RFFT(), (1, 1), None, T.Float32,
                        fftw_flags=pf._lib.FFTW_ESTIMATE)
 w = p64 * pf.allocate_input(p64)
+q64 = pf.PencilFFTPlan((64, 64, 64), T.RFFT(), (1, 1), None, T.Float32,
+                       flags=pf._lib.FLAG_NO_AUTOTUNE | pf._lib.FLAG_STREAM_STORES)
+w = q64 * pf.allocate_input(q64)
 torch.cuda.synchronize()
 t_first = time.perf_counter() - t0
 # planning: pfft_plan_create + pfft_plan_prepare (tables uploaded, kernels
