@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--extra", action="append", default=None,
-                    choices=["plain", "small_64_f64", "precision_1024", "batched_graph", "r2r"],
+                    choices=["plain", "small_64_f64", "precision_1024", "batched_graph", "r2r",
+                             "generic"],
                     help="run only the named extra workload(s)")
     ap.add_argument("--no-checks", action="store_true",
                     help="skip the Float64-plan error and the copy-ceiling measurement "
@@ -419,6 +420,27 @@ def extra_r2r(torch, pf, steps, warmup):
     return res
 
 
+def extra_generic(torch, pf, steps, warmup):
+    """Lengths without a specialised kernel: a prime line in the Bluestein
+    form, and lines with radix-7/11/13 direct stages."""
+    T = pf.Transforms
+    res = {}
+    g = torch.Generator(device="cuda").manual_seed(13)
+    for key, dims, label in (
+            ("bluestein_1031x128x128_c32", (1031, 128, 128),
+             "1031x128x128 ComplexF32 FFT+BFFT pair (1031-point lines: Bluestein)"),
+            ("primes_448x352x208_c32", (448, 352, 208),
+             "448x352x208 ComplexF32 FFT+BFFT pair (radices 7, 11, 13)")):
+        plan = pf.PencilFFTPlan(dims, T.FFT(), (1, 1), None, T.Float32)
+        u = pf.allocate_input(plan)
+        v = pf.allocate_output(plan)
+        fill(torch, u, g)
+        res[key] = pair_entry(torch, pf, plan, u, v, steps, warmup, label)
+        del plan, u, v
+        torch.cuda.empty_cache()
+    return res
+
+
 def extra_small_f64(torch, pf, steps, warmup):
     """The reference's CPU-runnable case: 64^3 Float64 field, uniform [0,1)
     with seed 42, RFFT then normalised BRFFT (ldiv!), forward and backward
@@ -553,7 +575,7 @@ def run_extras(torch, pf, steps, warmup, only=None):
     res = {}
     for key, fn in (("plain", extra_plain), ("small_64_f64", extra_small_f64),
                     ("precision_1024", extra_precision), ("batched_graph", extra_batched),
-                    ("r2r", extra_r2r)):
+                    ("r2r", extra_r2r), ("generic", extra_generic)):
         if only and key not in only:
             continue
         try:
