@@ -94,10 +94,12 @@ static int set_lds_limit(const void* fn, int64_t bytes) {
 // per wave).  The first use of a pass shape times every candidate on scratch
 // buffers of the pass's size and keeps the fastest; the choice is cached per
 // process by the pass signature and its place in the plan.  Within one line
-// length the variants differ only in how work is spread, so their results
-// are bit-identical; the exception is 1024-point lines, whose two-wavefront
-// variants (W = 2) run the 8.8.4.4 radix schedule instead of 16.16.4 and
-// agree with the others to rounding only.
+// length the variants differ in how work is spread, not in the arithmetic,
+// but their results agree to rounding only: 1024-point lines' two-wavefront
+// variants (W = 2) run the 8.8.4.4 radix schedule instead of 16.16.4, and
+// separate template instances may contract a multiply-add differently.  A
+// plan pinned by PFFT_FLAG_NO_AUTOTUNE (or by imported tune data) is
+// bit-reproducible.
 static std::mutex g_tune_mu;
 static std::map<std::string, int> g_tune_cache;
 
