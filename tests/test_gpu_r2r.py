@@ -190,3 +190,20 @@ def test_dct_xlane_candidates(pf, gpu, kind, data):
     xl = sorted(s for s in seen if ",xlane" in s and ("dct" in s))
     print(kind, data, xl)
     assert xl, sorted(seen)
+
+
+@pytest.mark.parametrize("permute,extra", [(False, ()), (True, (3,)), (False, (2,))],
+                         ids=["noperm", "extra3", "noperm-extra2"])
+@pytest.mark.parametrize("kind,n", [("REDFT00", 257), ("RODFT00", 255), ("REDFT11", 256),
+                                    ("RODFT11", 256)])
+def test_new_kinds_layouts(pf, gpu, kind, n, permute, extra):
+    """DCT-I / DST-I / DCT-IV / DST-IV passes under the layout options of
+    transforms.jl:414-462 (permute_dims=false) and with extra dims
+    (plans.jl:164-182), complex data, against the oracle."""
+    T = pf.Transforms
+    dims = (32, 32, n)
+    trs = (T.FFT(), T.FFT(), T.R2R(getattr(T, kind)))
+    tol = 5e-7 * np.log2(int(np.prod(dims) * int(np.prod(extra or (1,)))))
+    err_f, err_b, meta = run_case(pf, dims, trs, 0, permute=permute, extra=extra, seed=n)
+    print(kind, n, permute, extra, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= tol and err_b <= tol
