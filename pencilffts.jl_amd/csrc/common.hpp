@@ -111,7 +111,8 @@ struct PassArgs {
   // / tw describe the bs_M-point FFT.
   int64_t bs_M;
   const void* bs_chirp;   // c_j, j < M
-  const void* bs_bhat;    // bs_M-point FFT of b_m = conj(c_m) (m mod bs_M, |m| < M), / bs_M
+  const void* bs_bhat;    // bs_M-point FFT of b_m = conj(c_m) (m mod bs_M, |m| < M), / bs_M,
+                          // in the output order of the in-place forward FFT
   // First pass of the four-step split of a line too long for the LDS: output
   // k of the line with index j2 along line dim 0 is multiplied by
   // w^(j2 k), w = exp(-2 pi i / fs_M) (conjugated for a backward

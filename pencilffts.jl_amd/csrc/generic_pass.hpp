@@ -469,14 +469,65 @@ PFFT_HD void phase_stage(const PassArgs& a, const TileCtx& c, int s, int tid,
 }
 
 // ---- Bluestein form of the line FFT ----------------------------------------
-// The M-point transform of the pre-processed lines in work buffer 0, as
+// The M-point transform of the pre-processed lines in work buffer 0, all in
+// place there (one work buffer, so more workgroups fit on a CU):
 //   bs_pre   a_j = x_j c_j (x conjugated first for a backward transform),
-//            zero-padded to bs_M points, in place;
-//   stages   the bs_M-point forward FFT;
-//   bs_mul   times the transform of b (its 1 / bs_M included);
-//   stages   the bs_M-point backward FFT (the stage parity continues, so
-//            the result is back in work buffer 0 after the two FFTs);
+//            zero-padded to bs_M points;
+//   dif      the bs_M-point forward FFT, decimation in frequency over the
+//            radix schedule: bins land in digit-reversed positions;
+//   bs_mul   times the transform of b, stored in those positions (its
+//            1 / bs_M included);
+//   dit      the bs_M-point backward FFT, decimation in time over the
+//            stages in reverse: natural order again;
 //   bs_post  X_k = c_k C_k (conjugated back for a backward transform), k < M.
+//
+// In-place stage s of radix RAD over blocks of Bs = bs_M / (r_0 ... r_{s-1})
+// points: butterfly j < Bs / RAD of a block reads positions
+// j + q Bs / RAD (q < RAD).  Forward: DFT_RAD, then output q times
+// w_Bs^(q j).  Backward: input q times w_Bs^(-q j), then the conjugate DFT.
+template <typename R, int RAD>
+PFFT_HD void stage_inplace(const PassArgs& a, const TileCtx& c, int s, bool bwd, cpx<R>* W,
+                           int tid, int nthr) {
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  const uint32_t Mb = (uint32_t)a.bs_M;
+  uint32_t Bs = Mb;
+  for (int i = 0; i < s; ++i) Bs /= (uint32_t)a.radix[i];
+  const uint32_t stride = Bs / RAD, tstep = Mb / Bs;
+  const uint32_t nb = Mb / RAD;                      // butterflies per line
+  const uint32_t count = (uint32_t)c.Tw * nb;
+  // e = (w (Mb / Bs) + blk) stride + j
+  Digits<3> it(tid, nthr, {stride, Mb / Bs});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t j = it.d[0], blk = it.d[1], w = it.d[2];
+    cpx<R>* L = W + w * a.Mp;
+    const uint32_t base = blk * Bs + j;
+    cpx<R> v[RAD];
+    for (int q = 0; q < RAD; ++q) {
+      v[q] = L[wp(base + q * stride)];
+      if (bwd) v[q] = conj(v[q]);
+      if (bwd && q) v[q] = v[q] * tw[q * j * tstep];   // conj(v w^-qj)
+    }
+    if (RAD == 2) dft2(v);
+    else if (RAD == 4) dft4(v);
+    else dft8(v);
+    for (int q = 0; q < RAD; ++q) {
+      cpx<R> y = v[q];
+      if (!bwd && q) y = y * tw[q * j * tstep];
+      L[wp(base + q * stride)] = bwd ? conj(y) : y;
+    }
+  }
+}
+
+template <typename R>
+PFFT_HD void phase_stage_inplace(const PassArgs& a, const TileCtx& c, int s, bool bwd, int tid,
+                                 int nthr, char* lds) {
+  cpx<R>* W = (cpx<R>*)lds;
+  switch (a.radix[s]) {
+    case 2: stage_inplace<R, 2>(a, c, s, bwd, W, tid, nthr); break;
+    case 4: stage_inplace<R, 4>(a, c, s, bwd, W, tid, nthr); break;
+    default: stage_inplace<R, 8>(a, c, s, bwd, W, tid, nthr); break;
+  }
+}
 template <typename R>
 PFFT_HD void phase_bs_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr, char* lds) {
   cpx<R>* W = (cpx<R>*)lds;
@@ -498,8 +549,7 @@ PFFT_HD void phase_bs_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr
 
 template <typename R>
 PFFT_HD void phase_bs_mul(const PassArgs& a, const TileCtx& c, int tid, int nthr, char* lds) {
-  const int ns = a.nstages;
-  cpx<R>* W = (cpx<R>*)(lds + ((ns & 1) ? a.lds_w1_off : 0));
+  cpx<R>* W = (cpx<R>*)lds;
   const cpx<R>* bh = (const cpx<R>*)a.bs_bhat;
   const uint32_t Mb = (uint32_t)a.bs_M;
   const uint32_t count = (uint32_t)c.Tw * Mb;
@@ -539,10 +589,9 @@ PFFT_HD void phase_fft_step(const PassArgs& a, const TileCtx& c, int i, int tid,
   if (!a.bs_M) { phase_stage<R>(a, c, i, tid, nthr, lds); return; }
   const int ns = a.nstages;
   if (i == 0) phase_bs_pre<R>(a, c, tid, nthr, lds);
-  else if (i <= ns) phase_stage_ex<R>(a, c, i - 1, i - 1, a.bs_M, false, tid, nthr, lds);
+  else if (i <= ns) phase_stage_inplace<R>(a, c, i - 1, false, tid, nthr, lds);
   else if (i == ns + 1) phase_bs_mul<R>(a, c, tid, nthr, lds);
-  else if (i <= 2 * ns + 1)
-    phase_stage_ex<R>(a, c, i - ns - 2, i - 2, a.bs_M, true, tid, nthr, lds);
+  else if (i <= 2 * ns + 1) phase_stage_inplace<R>(a, c, 2 * ns + 1 - i, true, tid, nthr, lds);
   else phase_bs_post<R>(a, c, tid, nthr, lds);
 }
 

@@ -274,8 +274,25 @@ static int twiddle_table(Plan& p, int64_t M) {
 // the chirp c_j = exp(-i pi j^2 / M), j < M (argument j^2 mod 2M reduced
 // exactly), and the Mb-point DFT of b_m = conj(c_|m|) (|m| < M, index m mod
 // Mb, zero elsewhere) divided by Mb, from a radix-2 FFT in long double.
-static void bluestein_tables(Plan& p, int64_t M, int64_t Mb, int32_t* chirp_id,
-                             int32_t* bhat_id) {
+// The kernel's forward convolution FFT runs in place, decimation in frequency
+// over the radix schedule rad[0..ns): its output position
+// p = q0 Mb/r0 + q1 Mb/(r0 r1) + ... holds bin q0 + r0 (q1 + r1 (q2 + ...)).
+// The transform of b is stored in that order, so the pointwise product
+// needs no reordering and the in-place inverse (decimation in time) returns
+// natural order.
+static int64_t dif_bin(int64_t pos, int64_t Mb, const int32_t* rad, int ns) {
+  int64_t k = 0, w = 1, B = Mb;
+  for (int s = 0; s < ns; ++s) {
+    B /= rad[s];
+    const int64_t q = (pos / B) % rad[s];
+    k += q * w;
+    w *= rad[s];
+  }
+  return k;
+}
+
+static void bluestein_tables(Plan& p, int64_t M, int64_t Mb, const int32_t* rad, int ns,
+                             int32_t* chirp_id, int32_t* bhat_id) {
   std::vector<double> ch(2 * M);
   for (int64_t j = 0; j < M; ++j) expm((j * j) % (2 * M), 2 * M, &ch[2 * j], &ch[2 * j + 1]);
   std::vector<long double> re(Mb, 0.0L), im(Mb, 0.0L);
@@ -306,9 +323,10 @@ static void bluestein_tables(Plan& p, int64_t M, int64_t Mb, int32_t* chirp_id,
     }
   }
   std::vector<double> bh(2 * Mb);
-  for (int64_t k = 0; k < Mb; ++k) {
-    bh[2 * k] = (double)(re[k] / (long double)Mb);
-    bh[2 * k + 1] = (double)(im[k] / (long double)Mb);
+  for (int64_t pos = 0; pos < Mb; ++pos) {
+    const int64_t k = dif_bin(pos, Mb, rad, ns);
+    bh[2 * pos] = (double)(re[k] / (long double)Mb);
+    bh[2 * pos + 1] = (double)(im[k] / (long double)Mb);
   }
   *chirp_id = add_table(p, "bs_chirp", std::move(ch));
   *bhat_id = add_table(p, "bs_bhat", std::move(bh));
@@ -402,9 +420,10 @@ static int line_fft(Plan& p, PassPlan& pp) {
   if (largest_prime_factor(M) > kBluesteinPrime) {
     Mb = 1;
     while (Mb < 2 * M - 1) Mb *= 2;
+    // one work buffer (the convolution runs in place) beside the raw line
     const int64_t Tw1 = a.r2r_cplx ? 2 : 1;
     const int64_t w0 = (Tw1 * line_pitch_slots(Mb) * cs + 255) / 256 * 256;
-    if (w0 + std::max(Tw1 * line_pitch_slots(Mb) * cs, a.raw_pitch * rs) > kMaxLds) Mb = 0;
+    if (w0 + a.raw_pitch * rs > kMaxLds) Mb = 0;
   }
   const int ns = radix_schedule(Mb ? Mb : M, a.radix);
   if (ns < 0) return fail(PFFT_ERR_UNSUPPORTED, "too many radix stages");
@@ -412,7 +431,7 @@ static int line_fft(Plan& p, PassPlan& pp) {
   pp.tw_id = twiddle_table(p, Mb ? Mb : M);
   if (Mb) {
     a.bs_M = Mb;
-    bluestein_tables(p, M, Mb, &pp.bs_chirp_id, &pp.bs_bhat_id);
+    bluestein_tables(p, M, Mb, a.radix, ns, &pp.bs_chirp_id, &pp.bs_bhat_id);
   }
   // aux tables
   std::vector<double> aux;
@@ -513,7 +532,8 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   auto footprint = [&](int64_t T, int64_t* w1off) {
     const int64_t Tw = a.r2r_cplx ? 2 * T : T;
     const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
-    const int64_t w1 = (lk == L_COPY || a.nstages == 0) ? 0 : Tw * a.Mp * cs;
+    // (the Bluestein form's convolution runs in place in work buffer 0)
+    const int64_t w1 = (lk == L_COPY || a.nstages == 0 || a.bs_M) ? 0 : Tw * a.Mp * cs;
     const int64_t raw = T * a.raw_pitch * rs;
     *w1off = (w0 + 255) / 256 * 256;
     return *w1off + std::max(w1, raw);
