@@ -184,6 +184,24 @@ PFFT_HD void phase_load(const PassArgs& a, const TileCtx& c, int tid, int nthr,
   const uint32_t count = Tact * L;
   const bool along = (a.sk_in == 1);
   Digits<2> it(tid, nthr, {along ? L : Tact});
+  if (a.load_w0) {
+    // c2c: straight into the work lines (the pre phase has nothing to do),
+    // with the spectral operator of side 1
+    cpx<R>* W = (cpx<R>*)lds;
+    for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+      const uint32_t t = along ? it.d[1] : it.d[0];
+      const uint32_t j = along ? it.d[0] : it.d[1];
+      const int64_t g = c.in_base + (int64_t)t * a.lsin[0] + (int64_t)j * a.sk_in;
+      cpx<R> v = ((const cpx<R>*)a.in)[g];
+      if (a.op && a.op_side == 1) {
+        int64_t m0;
+        const OpLine<R> ot = op_tile<R>(a, c.tile, &m0);
+        v = v * op_factor<R>(a, op_line<R>(a, ot, m0 + t), (int)j);
+      }
+      W[t * a.Mp + wp(j)] = v;
+    }
+    return;
+  }
   for (uint32_t e = tid; e < count; e += nthr, it.next()) {
     const uint32_t t = along ? it.d[1] : it.d[0];
     const uint32_t j = along ? it.d[0] : it.d[1];
@@ -212,7 +230,7 @@ PFFT_HD cpx<R> raw_cplx(const PassArgs& a, const R* raw, uint32_t t, int64_t j) 
 template <typename R>
 PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
                        char* lds) {
-  if (a.kind == L_COPY) return;
+  if (a.kind == L_COPY || a.load_w0) return;
   const R* raw = (const R*)(lds + a.lds_w1_off);
   cpx<R>* W = (cpx<R>*)lds;
   const cpx<R>* aux = (const cpx<R>*)a.aux;

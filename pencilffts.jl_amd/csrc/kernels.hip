@@ -29,8 +29,10 @@ __global__ __launch_bounds__(NT) void k_generic(PassArgs a) {
     tile_init(a, tile, c);
     phase_load<R>(a, c, threadIdx.x, blockDim.x, lds);
     __syncthreads();
-    phase_pre<R>(a, c, threadIdx.x, blockDim.x, lds);
-    __syncthreads();
+    if (!a.load_w0) {
+      phase_pre<R>(a, c, threadIdx.x, blockDim.x, lds);
+      __syncthreads();
+    }
     const int nsteps = fft_steps(a);
     for (int s = 0; s < nsteps; ++s) {
       phase_fft_step<R>(a, c, s, threadIdx.x, blockDim.x, lds);

@@ -529,12 +529,14 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   // tile size and LDS footprint of the generic pass
   const int64_t Mw = a.bs_M ? a.bs_M : M;  // points per work line
   a.Mp = line_pitch_slots(Mw);
+  // c2c lines load straight into the work lines: no raw tile, no pre phase
+  a.load_w0 = lk == L_C2C ? 1 : 0;
   auto footprint = [&](int64_t T, int64_t* w1off) {
     const int64_t Tw = a.r2r_cplx ? 2 * T : T;
     const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
     // (the Bluestein form's convolution runs in place in work buffer 0)
     const int64_t w1 = (lk == L_COPY || a.nstages == 0 || a.bs_M) ? 0 : Tw * a.Mp * cs;
-    const int64_t raw = T * a.raw_pitch * rs;
+    const int64_t raw = a.load_w0 ? 0 : T * a.raw_pitch * rs;
     *w1off = (w0 + 255) / 256 * 256;
     return *w1off + std::max(w1, raw);
   };
