@@ -183,6 +183,21 @@ PFFT_HD void phase_load(const PassArgs& a, const TileCtx& c, int tid, int nthr,
   const uint32_t Tact = (uint32_t)c.Tact;
   const uint32_t count = Tact * L;
   const bool along = (a.sk_in == 1);
+  if (a.load_w0 && a.kind == L_R2C_HALF) {
+    // r2c of even length: the real pairs (x_2j, x_2j+1) straight into the
+    // work lines as complex points
+    cpx<R>* W = (cpx<R>*)lds;
+    const uint32_t M = (uint32_t)a.M;
+    const R* in = (const R*)a.in;
+    Digits<2> jt(tid, nthr, {along ? M : Tact});
+    for (uint32_t e = tid; e < Tact * M; e += nthr, jt.next()) {
+      const uint32_t t = along ? jt.d[1] : jt.d[0];
+      const uint32_t j = along ? jt.d[0] : jt.d[1];
+      const int64_t g = c.in_base + (int64_t)t * a.lsin[0] + (int64_t)(2 * j) * a.sk_in;
+      W[t * a.Mp + wp(j)] = mk(in[g], in[g + a.sk_in]);
+    }
+    return;
+  }
   Digits<2> it(tid, nthr, {along ? L : Tact});
   if (a.load_w0) {
     // c2c: straight into the work lines (the pre phase has nothing to do),
