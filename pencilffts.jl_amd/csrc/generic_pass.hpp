@@ -198,6 +198,29 @@ PFFT_HD void phase_load(const PassArgs& a, const TileCtx& c, int tid, int nthr,
     }
     return;
   }
+  if (a.load_w0 && a.kind == L_C2R_HALF) {
+    // c2r of even length: the Hermitian merge of X_j with X_{M-j}, both read
+    // from memory (each wavefront's reads run up, or down, one contiguous
+    // stretch; the second touch of a point comes from the caches)
+    cpx<R>* W = (cpx<R>*)lds;
+    const uint32_t M = (uint32_t)a.M;
+    const cpx<R>* in = (const cpx<R>*)a.in;
+    const cpx<R>* aux = (const cpx<R>*)a.aux;
+    Digits<2> jt(tid, nthr, {along ? M : Tact});
+    for (uint32_t e = tid; e < Tact * M; e += nthr, jt.next()) {
+      const uint32_t t = along ? jt.d[1] : jt.d[0];
+      const uint32_t j = along ? jt.d[0] : jt.d[1];
+      const int64_t g = c.in_base + (int64_t)t * a.lsin[0];
+      cpx<R> A = in[g + (int64_t)j * a.sk_in];
+      cpx<R> B = conj(in[g + (int64_t)(M - j) * a.sk_in]);
+      if (j == 0) { A.im = R(0); B.im = R(0); }  // Im of DC and Nyquist ignored
+      else if (a.adjoint) { A = scl(A, R(0.5)); B = scl(B, R(0.5)); }  // X_k / w_k
+      const cpx<R> E = A + B;
+      const cpx<R> O = (A - B) * conj(aux[j]);
+      W[t * a.Mp + wp(j)] = E + mul_pi(O);
+    }
+    return;
+  }
   Digits<2> it(tid, nthr, {along ? L : Tact});
   if (a.load_w0) {
     // c2c: straight into the work lines (the pre phase has nothing to do),

@@ -529,9 +529,9 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   // tile size and LDS footprint of the generic pass
   const int64_t Mw = a.bs_M ? a.bs_M : M;  // points per work line
   a.Mp = line_pitch_slots(Mw);
-  // c2c lines (and the real pairs of an even r2c) load straight into the
-  // work lines: no raw tile, no pre phase
-  a.load_w0 = (lk == L_C2C || (lk == L_R2C_HALF && !a.r2r_cplx)) ? 1 : 0;
+  // c2c lines (and the real pairs of an even r2c, the Hermitian merge of an
+  // even c2r) load straight into the work lines: no raw tile, no pre phase
+  a.load_w0 = (lk == L_C2C || lk == L_R2C_HALF || lk == L_C2R_HALF) ? 1 : 0;
   auto footprint = [&](int64_t T, int64_t* w1off) {
     const int64_t Tw = a.r2r_cplx ? 2 * T : T;
     const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
