@@ -171,7 +171,10 @@ PFFT_HD void tile_init(const PassArgs& a, int64_t tile, TileCtx& c) {
   c.Tact = (int32_t)(left < a.T ? left : a.T);
   c.tile = tile;
   c.m0 = b0 * a.T;
-  c.Tw = a.r2r_cplx ? 2 * c.Tact : c.Tact;
+  // real lines of the tile: two per line for complex R2R data; paired as
+  // the real and imaginary parts of one work line for DCT-I / DST-I
+  const int32_t nreal = a.r2r_cplx ? 2 * c.Tact : c.Tact;
+  c.Tw = a.pair_real ? (nreal + 1) / 2 : nreal;
 }
 
 // ---- phase 1: load -------------------------------------------------------
@@ -314,9 +317,14 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         if (a.adjoint && j != 0) v = scl(v, R(0.5));  // X_k / w_k
         break;
       }
-      case L_DCT1:
-        v = mk(raw_real(a, raw, w, j < n ? j : M - j), zero);
+      case L_DCT1: {
+        // even extension of real lines 2w and 2w + 1 (paired)
+        const int64_t src = j < n ? j : M - j;
+        const uint32_t nreal = a.r2r_cplx ? 2 * (uint32_t)c.Tact : (uint32_t)c.Tact;
+        v = mk(raw_real(a, raw, 2 * w, src),
+               2 * w + 1 < nreal ? raw_real(a, raw, 2 * w + 1, src) : zero);
         break;
+      }
       case L_DCT2:
       case L_DST2: {
         const int64_t h = (n + 1) / 2;
@@ -359,10 +367,18 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         }
         break;
       case L_DST1: {
+        // odd extension of real lines 2w and 2w + 1 (paired)
         const int64_t jj = j;
-        if (jj == 0 || jj == n + 1) v = mk(zero, zero);
-        else if (jj <= n) v = mk(raw_real(a, raw, w, jj - 1), zero);
-        else v = mk(-raw_real(a, raw, w, (int64_t)M - jj - 1), zero);
+        const uint32_t nreal = a.r2r_cplx ? 2 * (uint32_t)c.Tact : (uint32_t)c.Tact;
+        const bool hi = 2 * w + 1 < nreal;
+        if (jj == 0 || jj == n + 1) {
+          v = mk(zero, zero);
+        } else if (jj <= n) {
+          v = mk(raw_real(a, raw, 2 * w, jj - 1), hi ? raw_real(a, raw, 2 * w + 1, jj - 1) : zero);
+        } else {
+          const int64_t src = (int64_t)M - jj - 1;
+          v = mk(-raw_real(a, raw, 2 * w, src), hi ? -raw_real(a, raw, 2 * w + 1, src) : zero);
+        }
         break;
       }
       default:
@@ -659,8 +675,12 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
   const int64_t n = a.n;
   switch (a.kind) {
     case L_C2R_HALF: return (k & 1) ? L[wp((uint32_t)(k >> 1))].im : L[wp((uint32_t)(k >> 1))].re;
-    case L_C2R_FULL:
-    case L_DCT1: return L[wp((uint32_t)(k))].re;
+    case L_C2R_FULL: return L[wp((uint32_t)(k))].re;
+    case L_DCT1: {
+      // real line w is part w & 1 of work line w >> 1
+      const cpx<R> z = (W + (w >> 1) * a.Mp)[wp((uint32_t)k)];
+      return (w & 1) ? z.im : z.re;
+    }
     case L_DCT2: {
       const cpx<R> z = aux[k] * L[wp((uint32_t)(k))];
       return R(2) * z.re;
@@ -691,7 +711,11 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
       const R y = (k & 1) ? -z.im : z.re;
       return (a.kind == L_DST4H && (k & 1)) ? -y : y;
     }
-    case L_DST1: return -L[wp((uint32_t)(k + 1))].im;
+    case L_DST1: {
+      // the odd extension's transform F of (a + i b) is S_b - i S_a
+      const cpx<R> z = (W + (w >> 1) * a.Mp)[wp((uint32_t)(k + 1))];
+      return (w & 1) ? z.re : -z.im;
+    }
     case L_DHT: return L[wp((uint32_t)(k))].re - L[wp((uint32_t)(k))].im;
     default: return R(0);
   }

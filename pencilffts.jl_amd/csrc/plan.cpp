@@ -532,8 +532,12 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   // c2c lines (and the real pairs of an even r2c, the Hermitian merge of an
   // even c2r) load straight into the work lines: no raw tile, no pre phase
   a.load_w0 = (lk == L_C2C || lk == L_R2C_HALF || lk == L_C2R_HALF) ? 1 : 0;
+  // DCT-I / DST-I lines are real sequences with real transforms: two of
+  // them share one complex FFT (real and imaginary parts)
+  a.pair_real = (lk == L_DCT1 || lk == L_DST1) ? 1 : 0;
   auto footprint = [&](int64_t T, int64_t* w1off) {
-    const int64_t Tw = a.r2r_cplx ? 2 * T : T;
+    const int64_t nreal = a.r2r_cplx ? 2 * T : T;
+    const int64_t Tw = a.pair_real ? (nreal + 1) / 2 : nreal;
     const int64_t w0 = lk == L_COPY ? 0 : Tw * a.Mp * cs;
     // (the Bluestein form's convolution runs in place in work buffer 0)
     const int64_t w1 = (lk == L_COPY || a.nstages == 0 || a.bs_M) ? 0 : Tw * a.Mp * cs;
