@@ -478,7 +478,57 @@ PFFT_HD void stage_fixed(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_
   }
 }
 
-// Any radix (e.g. a prime factor 7, 11, 41): one work item per output.
+// Prime radix 7, 11 or 13: one butterfly per work item (the RAD inputs
+// read once), the DFT in the symmetric form
+//   y_0 = v_0 + sum_r A_r,  y_q, y_{RAD-q} = P_q -+ i Q_q,
+//   P_q = v_0 + sum_r A_r cos(2 pi q r / RAD),  Q_q = sum_r B_r sin(2 pi q r / RAD),
+// A_r = v_r + v_{RAD-r}, B_r = v_r - v_{RAD-r} (r, q = 1 .. (RAD-1)/2); the
+// cosines and sines are the table's w_M^(m M / RAD) = cos - i sin.
+template <typename R, int RAD>
+PFFT_HD void stage_prime(const PassArgs& a, const TileCtx& c, int64_t Ns, int64_t fftM, bool bwd,
+                         const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
+  constexpr int H = (RAD - 1) / 2;
+  const cpx<R>* tw = (const cpx<R>*)a.tw;
+  const uint32_t M = (uint32_t)fftM;
+  const uint32_t nb = M / RAD;
+  const uint32_t step = (uint32_t)(M / (Ns * RAD));
+  const uint32_t count = (uint32_t)c.Tw * nb;
+  Digits<3> it(tid, nthr, {(uint32_t)Ns, nb / (uint32_t)Ns});
+  for (uint32_t e = tid; e < count; e += nthr, it.next()) {
+    const uint32_t jj = it.d[0], jq = it.d[1], w = it.d[2];
+    const uint32_t j = jq * (uint32_t)Ns + jj;
+    const cpx<R>* s = src + w * a.Mp;
+    cpx<R> v[RAD];
+    for (int r = 0; r < RAD; ++r) {
+      cpx<R> x = s[wp(j + r * nb)];
+      if (bwd) x = conj(x);
+      v[r] = r == 0 ? x : x * tw[r * jj * step];
+    }
+    cpx<R> A[H + 1], B[H + 1];
+    cpx<R> y0 = v[0];
+    for (int r = 1; r <= H; ++r) {
+      A[r] = v[r] + v[RAD - r];
+      B[r] = v[r] - v[RAD - r];
+      y0 = y0 + A[r];
+    }
+    cpx<R>* d = dst + w * a.Mp;
+    const uint32_t ob = jq * (uint32_t)Ns * RAD + jj;
+    d[wp(ob)] = bwd ? conj(y0) : y0;
+    for (int q = 1; q <= H; ++q) {
+      cpx<R> P = v[0], Q = mk(R(0), R(0));
+      for (int r = 1; r <= H; ++r) {
+        const cpx<R> wm = tw[((q * r) % RAD) * nb];   // cos - i sin
+        P = P + scl(A[r], wm.re);
+        Q = Q + scl(B[r], -wm.im);
+      }
+      const cpx<R> lo = P + mul_mi(Q), hi = P - mul_mi(Q);   // P -+ i Q
+      d[wp(ob + q * (uint32_t)Ns)] = bwd ? conj(lo) : lo;
+      d[wp(ob + (RAD - q) * (uint32_t)Ns)] = bwd ? conj(hi) : hi;
+    }
+  }
+}
+
+// Any radix (e.g. a prime factor 17, 41): one work item per output.
 template <typename R>
 PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD, int64_t fftM,
                        bool bwd, const cpx<R>* src, cpx<R>* dst, int tid, int nthr) {
@@ -516,7 +566,10 @@ PFFT_HD void stage_any(const PassArgs& a, const TileCtx& c, int64_t Ns, int RAD,
 
 // Stage s of the line FFT (fftM points, radix schedule a.radix), from the
 // work buffer of parity sb to the other one; bwd: backward butterflies.
-template <typename R>
+// PRIME: radix 7, 11 and 13 as butterflies (stage_prime; its registers set
+// the kernel's, so only the instance for lines with such stages has it),
+// else as one work item per output (stage_any)
+template <typename R, bool PRIME = true>
 PFFT_HD void phase_stage_ex(const PassArgs& a, const TileCtx& c, int s, int sb, int64_t fftM,
                             bool bwd, int tid, int nthr, char* lds) {
   int64_t Ns = 1;
@@ -529,15 +582,22 @@ PFFT_HD void phase_stage_ex(const PassArgs& a, const TileCtx& c, int s, int sb, 
     case 4: stage_fixed<R, 4>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
     case 5: stage_fixed<R, 5>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
     case 8: stage_fixed<R, 8>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break;
-    default: stage_any<R>(a, c, Ns, a.radix[s], fftM, bwd, src, dst, tid, nthr); break;
+    default:
+      if constexpr (PRIME) {
+        if (a.radix[s] == 7) { stage_prime<R, 7>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break; }
+        if (a.radix[s] == 11) { stage_prime<R, 11>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break; }
+        if (a.radix[s] == 13) { stage_prime<R, 13>(a, c, Ns, fftM, bwd, src, dst, tid, nthr); break; }
+      }
+      stage_any<R>(a, c, Ns, a.radix[s], fftM, bwd, src, dst, tid, nthr);
+      break;
   }
 }
 
 // Stage s of the direct M-point line FFT.
-template <typename R>
+template <typename R, bool PRIME = true>
 PFFT_HD void phase_stage(const PassArgs& a, const TileCtx& c, int s, int tid,
                          int nthr, char* lds) {
-  phase_stage_ex<R>(a, c, s, s, a.M, a.bwd != 0, tid, nthr, lds);
+  phase_stage_ex<R, PRIME>(a, c, s, s, a.M, a.bwd != 0, tid, nthr, lds);
 }
 
 // ---- Bluestein form of the line FFT ----------------------------------------
@@ -655,10 +715,10 @@ PFFT_HD int fft_steps(const PassArgs& a) {
   return a.bs_M ? 2 * a.nstages + 3 : a.nstages;
 }
 
-template <typename R>
+template <typename R, bool PRIME = true>
 PFFT_HD void phase_fft_step(const PassArgs& a, const TileCtx& c, int i, int tid, int nthr,
                             char* lds) {
-  if (!a.bs_M) { phase_stage<R>(a, c, i, tid, nthr, lds); return; }
+  if (!a.bs_M) { phase_stage<R, PRIME>(a, c, i, tid, nthr, lds); return; }
   const int ns = a.nstages;
   if (i == 0) phase_bs_pre<R>(a, c, tid, nthr, lds);
   else if (i <= ns) phase_stage_inplace<R>(a, c, i - 1, false, tid, nthr, lds);

@@ -20,8 +20,10 @@
 namespace pfft {
 
 // ---- generic LDS pass ---------------------------------------------------------
-// (NT: the workgroup size it is compiled for; the launch uses exactly that)
-template <typename R, int NT>
+// (NT: the workgroup size it is compiled for; the launch uses exactly that.
+// PRIME: with the radix-7/11/13 butterflies, whose registers would cost
+// every other pass occupancy; used for lines that have such a stage)
+template <typename R, int NT, bool PRIME>
 __global__ __launch_bounds__(NT) void k_generic(PassArgs a) {
   extern __shared__ __align__(16) char lds[];
   for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -35,7 +37,7 @@ __global__ __launch_bounds__(NT) void k_generic(PassArgs a) {
     }
     const int nsteps = fft_steps(a);
     for (int s = 0; s < nsteps; ++s) {
-      phase_fft_step<R>(a, c, s, threadIdx.x, blockDim.x, lds);
+      phase_fft_step<R, PRIME>(a, c, s, threadIdx.x, blockDim.x, lds);
       __syncthreads();
     }
     phase_post<R>(a, c, threadIdx.x, blockDim.x, lds);
@@ -43,30 +45,43 @@ __global__ __launch_bounds__(NT) void k_generic(PassArgs a) {
   }
 }
 
-// the generic kernel instance of a pass (element type, workgroup size)
-static const void* generic_fn(int real_type, int threads) {
-  if (real_type == PFFT_F32)
-    return threads == kBlockThreadsHeavy ? (const void*)k_generic<float, kBlockThreadsHeavy>
-                                         : (const void*)k_generic<float, kBlockThreads>;
-  return threads == kBlockThreadsHeavy ? (const void*)k_generic<double, kBlockThreadsHeavy>
-                                       : (const void*)k_generic<double, kBlockThreads>;
+// lines with a radix-7, 11 or 13 stage (the butterfly instance)
+static bool prime_stages(const PassArgs& a) {
+  if (a.bs_M) return false;
+  for (int i = 0; i < a.nstages; ++i)
+    if (a.radix[i] == 7 || a.radix[i] == 11 || a.radix[i] == 13) return true;
+  return false;
 }
 
-static void launch_generic(const PassPlan& pp, int real_type, const PassArgs& a,
-                           hipStream_t st) {
+// the generic kernel instance of a pass (element type, workgroup size,
+// prime butterflies)
+template <typename R>
+static const void* generic_fn_t(int threads, bool prime) {
+  if (prime) return (const void*)k_generic<R, kBlockThreadsHeavy, true>;
+  return threads == kBlockThreadsHeavy ? (const void*)k_generic<R, kBlockThreadsHeavy, false>
+                                       : (const void*)k_generic<R, kBlockThreads, false>;
+}
+static const void* generic_fn(int real_type, const PassPlan& pp) {
+  const bool prime = prime_stages(pp.args) && pp.threads == kBlockThreadsHeavy;
+  return real_type == PFFT_F32 ? generic_fn_t<float>(pp.threads, prime)
+                               : generic_fn_t<double>(pp.threads, prime);
+}
+
+template <typename R>
+static void launch_generic_t(const PassPlan& pp, const PassArgs& a, hipStream_t st) {
   const dim3 g((unsigned)pp.grid), b(pp.threads);
   const size_t lds = (size_t)pp.lds_bytes;
-  if (real_type == PFFT_F32) {
-    if (pp.threads == kBlockThreadsHeavy)
-      hipLaunchKernelGGL((k_generic<float, kBlockThreadsHeavy>), g, b, lds, st, a);
-    else
-      hipLaunchKernelGGL((k_generic<float, kBlockThreads>), g, b, lds, st, a);
-  } else {
-    if (pp.threads == kBlockThreadsHeavy)
-      hipLaunchKernelGGL((k_generic<double, kBlockThreadsHeavy>), g, b, lds, st, a);
-    else
-      hipLaunchKernelGGL((k_generic<double, kBlockThreads>), g, b, lds, st, a);
-  }
+  if (pp.threads == kBlockThreadsHeavy && prime_stages(a))
+    hipLaunchKernelGGL((k_generic<R, kBlockThreadsHeavy, true>), g, b, lds, st, a);
+  else if (pp.threads == kBlockThreadsHeavy)
+    hipLaunchKernelGGL((k_generic<R, kBlockThreadsHeavy, false>), g, b, lds, st, a);
+  else
+    hipLaunchKernelGGL((k_generic<R, kBlockThreads, false>), g, b, lds, st, a);
+}
+static void launch_generic(const PassPlan& pp, int real_type, const PassArgs& a,
+                           hipStream_t st) {
+  if (real_type == PFFT_F32) launch_generic_t<float>(pp, a, st);
+  else launch_generic_t<double>(pp, a, st);
 }
 
 #define HIP_TRY(expr)                                                             \
@@ -220,7 +235,7 @@ static int launch_pass(const Plan& p, const PassPlan& pp, const void* src, void*
 
 static int prepare_pass(const Plan& p, const PassPlan& pp) {
   if (pp.variant != 0) return fast_prepare(pp);
-  return set_lds_limit(generic_fn(p.real_type, pp.threads), pp.lds_bytes);
+  return set_lds_limit(generic_fn(p.real_type, pp), pp.lds_bytes);
 }
 
 // Second tuning phase: every plain pass's leading candidates again, judged by
@@ -727,7 +742,7 @@ int device_prepare(Plan& p) {
   for (auto* list : {&p.fwd, &p.bwd}) {
     for (PassPlan& pp : *list) {
       if (pp.variant == 0) {
-        r = set_lds_limit(generic_fn(p.real_type, pp.threads), pp.lds_bytes);
+        r = set_lds_limit(generic_fn(p.real_type, pp), pp.lds_bytes);
       } else {
         r = fast_prepare(pp);
       }
