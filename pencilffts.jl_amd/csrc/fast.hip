@@ -1952,6 +1952,15 @@ static void add_mixed_chunks(std::vector<Entry>& v) {
   add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
 }
 
+// DCT-II / DCT-III (the Chebyshev pair) on the mixed-radix FP32 lengths
+template <int M>
+static void add_dct_mixed(std::vector<Entry>& v) {
+  add_r2r_kind<float, M, K_DCT2C, true>(v);
+  add_r2r_kind<float, M, K_DCT3C, true>(v);
+  add_r2r_kind<float, M, K_DCT2C | K_REAL_PAIRS, true>(v);
+  add_r2r_kind<float, M, K_DCT3C | K_REAL_PAIRS, true>(v);
+}
+
 // 256- and 512-point DCT-II / III / IV lines with the cross-lane exchange
 template <int M>
 static void add_dct_xlane(std::vector<Entry>& v) {
@@ -2049,6 +2058,11 @@ static const std::vector<Entry>& registry() {
     add_size<float, 768>(r);
     add_mixed_chunks<640>(r);
     add_mixed_chunks<768>(r);
+    add_dct_mixed<96>(r);
+    add_dct_mixed<160>(r);
+    add_dct_mixed<192>(r);
+    add_dct_mixed<320>(r);
+    add_dct_mixed<384>(r);
     return r;
   }();
   return v;

@@ -207,3 +207,19 @@ def test_new_kinds_layouts(pf, gpu, kind, n, permute, extra):
     err_f, err_b, meta = run_case(pf, dims, trs, 0, permute=permute, extra=extra, seed=n)
     print(kind, n, permute, extra, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
     assert err_f <= tol and err_b <= tol
+
+
+@pytest.mark.parametrize("data", ["real", "complex"])
+@pytest.mark.parametrize("dims", [(96, 192, 64), (384, 160, 32), (64, 320, 96)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_dct_mixed_lengths(pf, gpu, dims, data):
+    """REDFT10 / REDFT01 at the mixed-radix lengths 96, 160, 192, 320 and 384
+    (the Chebyshev pair on 3*2^k and 5*2^k points) on the specialised
+    kernels, against the oracle."""
+    T = pf.Transforms
+    err_f, err_b, meta = run_case(pf, dims, _trs(T, "REDFT10", data), 0, seed=3)
+    tol = 5e-7 * np.log2(int(np.prod(dims)))
+    print(dims, data, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= tol and err_b <= tol
+    mixed = [k for k in meta.kernels if ",dct" in k and any(",%d," % m in k for m in (96, 160, 192, 320, 384))]
+    assert mixed, meta.kernels
