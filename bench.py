@@ -380,7 +380,9 @@ def extra_plain(torch, pf, steps, warmup):
             ("rfft_256_f64", (256,) * 3, T.RFFT(), T.Float64, "256^3 Float64 RFFT+BRFFT pair"),
             ("mixed_384x384x256_f32", (384, 384, 256),
              (T.RFFT(), T.FFT(), T.R2R(T.REDFT10)), T.Float32,
-             "384x384x256 Float32 (RFFT, FFT, REDFT10) pair")):
+             "384x384x256 Float32 (RFFT, FFT, REDFT10) pair"),
+            ("rfft_1024_f64", (1024,) * 3, T.RFFT(), T.Float64,
+             "1024^3 Float64 RFFT+BRFFT pair")):
         plan = pf.PencilFFTPlan(dims, tr, (1, 1), None, rt)
         u = pf.allocate_input(plan)
         v = pf.allocate_output(plan)

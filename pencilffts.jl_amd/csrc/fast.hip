@@ -1952,6 +1952,15 @@ static void add_mixed_chunks(std::vector<Entry>& v) {
   add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
 }
 
+// Float64 1024-point c2c lines with a transposed side: 4-line tiles (64-byte
+// runs), since the line slots of 1024 complex doubles (18 KB each) and the
+// tile share the LDS
+static void add_f64_1024(std::vector<Entry>& v) {
+  add<double, 1024, 4, 1, K_C2C, IO_LINE, IO_TILE>(v);
+  add<double, 1024, 4, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<double, 1024, 4, 1, K_C2C, IO_TILE, IO_LINE>(v);
+}
+
 // DCT-II / DCT-III (the Chebyshev pair) on the mixed-radix FP32 lengths
 template <int M>
 static void add_dct_mixed(std::vector<Entry>& v) {
@@ -2063,6 +2072,8 @@ static const std::vector<Entry>& registry() {
     add_dct_mixed<192>(r);
     add_dct_mixed<320>(r);
     add_dct_mixed<384>(r);
+    add_size<double, 1024>(r);
+    add_f64_1024(r);
     return r;
   }();
   return v;

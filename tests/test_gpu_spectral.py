@@ -30,8 +30,7 @@ OPS = [(1, 0), (1, 1), (1, 2), (2, 0), (3, 0), (4, 0)]
 @pytest.mark.parametrize("real_type", [0, 1], ids=["F32", "F64"])
 @pytest.mark.parametrize("dims", SHAPES, ids=lambda d: "x".join(map(str, d)))
 def test_operators_forward_ldiv(pf, gpu, dims, real_type):
-    # Float64 lines of 1024 points run (and carry the operators) on the
-    # generic pass
+    # (1024, 32, 16): the r2c of 1024 reals, a 512-point complex FFT per line
     T = pf.Transforms
     S = pf.spectral
     meta, _ = meta_for(T, dims, (T.RFFT(),), CODE_NAMES[real_type])
