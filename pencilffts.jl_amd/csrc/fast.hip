@@ -1961,13 +1961,14 @@ static void add_f64_1024(std::vector<Entry>& v) {
   add<double, 1024, 4, 1, K_C2C, IO_TILE, IO_LINE>(v);
 }
 
-// DCT-II / DCT-III (the Chebyshev pair) on the mixed-radix FP32 lengths
-template <int M>
+// DCT-II / DCT-III (the Chebyshev pair) on the mixed-radix FP32 lengths, and
+// on the Float64 lengths without the full R2R set
+template <int M, typename R = float>
 static void add_dct_mixed(std::vector<Entry>& v) {
-  add_r2r_kind<float, M, K_DCT2C, true>(v);
-  add_r2r_kind<float, M, K_DCT3C, true>(v);
-  add_r2r_kind<float, M, K_DCT2C | K_REAL_PAIRS, true>(v);
-  add_r2r_kind<float, M, K_DCT3C | K_REAL_PAIRS, true>(v);
+  add_r2r_kind<R, M, K_DCT2C, true>(v);
+  add_r2r_kind<R, M, K_DCT3C, true>(v);
+  add_r2r_kind<R, M, K_DCT2C | K_REAL_PAIRS, true>(v);
+  add_r2r_kind<R, M, K_DCT3C | K_REAL_PAIRS, true>(v);
 }
 
 // 256- and 512-point DCT-II / III / IV lines with the cross-lane exchange
@@ -2074,6 +2075,8 @@ static const std::vector<Entry>& registry() {
     add_dct_mixed<384>(r);
     add_size<double, 1024>(r);
     add_f64_1024(r);
+    add_dct_mixed<128, double>(r);
+    add_dct_mixed<512, double>(r);
     return r;
   }();
   return v;

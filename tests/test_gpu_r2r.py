@@ -223,3 +223,16 @@ def test_dct_mixed_lengths(pf, gpu, dims, data):
     assert err_f <= tol and err_b <= tol
     mixed = [k for k in meta.kernels if ",dct" in k and any(",%d," % m in k for m in (96, 160, 192, 320, 384))]
     assert mixed, meta.kernels
+
+
+@pytest.mark.parametrize("data", ["real", "complex"])
+def test_dct_f64_128_512(pf, gpu, data):
+    """Float64 REDFT10 / REDFT01 on 512- and 128-point lines (the specialised
+    DCT-II / DCT-III kernels), against the oracle at the Float64 bound."""
+    T = pf.Transforms
+    dims = (512, 128, 32) if data == "real" else (32, 512, 128)
+    err_f, err_b, meta = run_case(pf, dims, _trs(T, "REDFT10", data), 1, seed=9)
+    print(dims, data, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= 1e-13 and err_b <= 1e-13
+    assert any(k.startswith("k_fast<f64,512,dct") for k in meta.kernels), meta.kernels
+    assert any(k.startswith("k_fast<f64,128,dct") for k in meta.kernels), meta.kernels
