@@ -2077,6 +2077,9 @@ static const std::vector<Entry>& registry() {
     add_f64_1024(r);
     add_dct_mixed<128, double>(r);
     add_dct_mixed<512, double>(r);
+    add_size<double, 96>(r);
+    add_size<double, 192>(r);
+    add_size<double, 384>(r);
     return r;
   }();
   return v;

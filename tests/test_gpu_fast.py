@@ -155,3 +155,15 @@ def test_prefetching_tile_loads(pf, gpu, dims, real_type, permute):
             base = lambda s: s.replace(",rev", "").replace(",nt", "")
             assert any(base(x) == base(name) for x in meta.kernels), (name, meta.kernels)
             assert err_f <= tol and err_b <= tol, (name, rev, err_f, err_b)
+
+
+@pytest.mark.parametrize("tname", ["RFFT", "FFT"])
+@pytest.mark.parametrize("dims", [(384, 96, 64), (192, 384, 32)], ids=lambda d: "x".join(map(str, d)))
+def test_fast_mixed_radix_f64(pf, gpu, dims, tname):
+    """Float64 lines of 96, 192 and 384 points (also as the half-length FFTs
+    of 384 reals) on the specialised kernels, at the Float64 bound."""
+    T = pf.Transforms
+    err_f, err_b, meta = run_case(pf, dims, (getattr(T, tname)(),), 1, seed=6)
+    print(dims, tname, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= 1e-13 and err_b <= 1e-13
+    assert all(k.startswith("k_fast<f64") for k in meta.kernels), meta.kernels
