@@ -378,6 +378,20 @@ template <> struct Cfg<1024> {
 // by workgroup barriers.  (Twice the lanes on 128- and 256-point lines, at 4
 // points per lane with radix-4 stages, measured 5-25 % slower than W = 1:
 // one more LDS round trip per line outweighs the extra waves.)
+template <> struct Cfg<2048> {
+  static constexpr int L = 64, P = 32;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 2048, L, P, 1, true, 16, 16, 8>(v, line, l, tw);
+  }
+};
+template <> struct Cfg<2048, 2> {
+  static constexpr int L = 128, P = 16;
+  template <typename R>
+  __device__ static void fft(C<R>* v, C<R>* line, int l, const C<R>* tw) {
+    fft_rec<R, 2048, L, P, 1, true, 16, 16, 8>(v, line, l, tw);
+  }
+};
 template <> struct Cfg<1024, 2> {
   static constexpr int L = 128, P = 8;
   template <typename R>
@@ -1952,6 +1966,19 @@ static void add_mixed_chunks(std::vector<Entry>& v) {
   add<float, M, 16, 1, K_R2C, IO_LINE, IO_TILE, 0, 4>(v);
 }
 
+// 2048-point FP32 c2c lines (32 points per lane, or 16 with two wavefronts
+// per line): 4-line tiles, since the line slots (18 KB each) and the tile
+// share the LDS
+static void add_2048(std::vector<Entry>& v) {
+  add<float, 2048, 4, 1, K_C2C, IO_LINE, IO_LINE>(v);
+  add<float, 2048, 2, 1, K_C2C, IO_LINE, IO_LINE, 0, 1, 2>(v);
+  add<float, 2048, 4, 1, K_C2C, IO_LINE, IO_TILE>(v);
+  add<float, 2048, 4, 1, K_C2C, IO_LINE, IO_TILE, 0, 2>(v);
+  add<float, 2048, 4, 1, K_C2C, IO_LINE, IO_TILE, 0, 2, 2>(v);
+  add<float, 2048, 4, 1, K_C2C, IO_TILE, IO_LINE>(v);
+  add<float, 2048, 4, 1, K_C2C, IO_TILE, IO_LINE, 0, 1, 2>(v);
+}
+
 // Float64 1024-point c2c lines with a transposed side: 4-line tiles (64-byte
 // runs), since the line slots of 1024 complex doubles (18 KB each) and the
 // tile share the LDS
@@ -2080,6 +2107,7 @@ static const std::vector<Entry>& registry() {
     add_size<double, 96>(r);
     add_size<double, 192>(r);
     add_size<double, 384>(r);
+    add_2048(r);
     return r;
   }();
   return v;

@@ -167,3 +167,19 @@ def test_fast_mixed_radix_f64(pf, gpu, dims, tname):
     print(dims, tname, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
     assert err_f <= 1e-13 and err_b <= 1e-13
     assert all(k.startswith("k_fast<f64") for k in meta.kernels), meta.kernels
+
+
+@pytest.mark.parametrize("permute", [True, False], ids=["perm", "noperm"])
+@pytest.mark.parametrize("dims", [(2048, 32, 16), (16, 2048, 32), (4096, 16, 8)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_fast_2048(pf, gpu, dims, permute):
+    """2048-point FP32 c2c lines (FFT, and the r2c of 4096 reals' other dims)
+    against the oracle; permuted, their passes run specialised kernels."""
+    T = pf.Transforms
+    tr = T.RFFT() if dims[0] == 4096 else T.FFT()
+    err_f, err_b, meta = run_case(pf, dims, (tr,), 0, permute=permute, seed=8)
+    tol = 5e-7 * np.log2(int(np.prod(dims)))
+    print(dims, permute, meta.kernels, "fwd %.2e rt %.2e" % (err_f, err_b))
+    assert err_f <= tol and err_b <= tol
+    if permute and dims[0] != 4096:
+        assert sum(",2048,c2c" in k for k in meta.kernels) == 2, meta.kernels
