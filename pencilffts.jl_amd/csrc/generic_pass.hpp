@@ -327,24 +327,32 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
       }
       case L_DCT2:
       case L_DST2: {
+        // Makhoul reorder of real lines 2w and 2w + 1 (paired): their
+        // transforms are separated by the Hermitian split in the post phase
         const int64_t h = (n + 1) / 2;
-        if ((int64_t)j < h) v = mk(raw_real(a, raw, w, 2 * j), zero);
-        else {
-          const R x = raw_real(a, raw, w, 2 * n - 2 * (int64_t)j - 1);
-          v = mk(a.kind == L_DST2 ? -x : x, zero);
-        }
+        const uint32_t nreal = a.r2r_cplx ? 2 * (uint32_t)c.Tact : (uint32_t)c.Tact;
+        const bool hi = 2 * w + 1 < nreal;
+        const int64_t src = (int64_t)j < h ? 2 * (int64_t)j : 2 * n - 2 * (int64_t)j - 1;
+        const R sg = ((int64_t)j >= h && a.kind == L_DST2) ? R(-1) : R(1);
+        v = mk(sg * raw_real(a, raw, 2 * w, src), hi ? sg * raw_real(a, raw, 2 * w + 1, src) : zero);
         break;
       }
-      case L_DCT3: {
-        const R xk = raw_real(a, raw, w, j);
-        const R xnk = j == 0 ? zero : raw_real(a, raw, w, n - j);
-        v = conj(aux[j]) * mk(xk, -xnk);
-        break;
-      }
+      case L_DCT3:
       case L_DST3: {
-        const R xk = raw_real(a, raw, w, n - 1 - j);
-        const R xnk = j == 0 ? zero : raw_real(a, raw, w, (int64_t)j - 1);
-        v = conj(aux[j]) * mk(xk, -xnk);
+        // V_j = w_4n^-j (x_j - i x_{n-j}) of real lines 2w and 2w + 1, as
+        // V_a + i V_b (V Hermitian: the transform is real, one line per part)
+        const uint32_t nreal = a.r2r_cplx ? 2 * (uint32_t)c.Tact : (uint32_t)c.Tact;
+        const bool hi = 2 * w + 1 < nreal;
+        const bool dst = a.kind == L_DST3;
+        const int64_t ik = dst ? n - 1 - (int64_t)j : (int64_t)j;
+        const int64_t ink = dst ? (int64_t)j - 1 : n - (int64_t)j;
+        const cpx<R> wj = conj(aux[j]);
+        const cpx<R> va = wj * mk(raw_real(a, raw, 2 * w, ik),
+                                  j == 0 ? zero : -raw_real(a, raw, 2 * w, ink));
+        const cpx<R> vb = hi ? wj * mk(raw_real(a, raw, 2 * w + 1, ik),
+                                       j == 0 ? zero : -raw_real(a, raw, 2 * w + 1, ink))
+                             : mk(zero, zero);
+        v = va + mul_pi(vb);
         break;
       }
       case L_DCT4:
@@ -741,19 +749,23 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
       const cpx<R> z = (W + (w >> 1) * a.Mp)[wp((uint32_t)k)];
       return (w & 1) ? z.im : z.re;
     }
-    case L_DCT2: {
-      const cpx<R> z = aux[k] * L[wp((uint32_t)(k))];
-      return R(2) * z.re;
-    }
+    case L_DCT2:
     case L_DST2: {
-      const int64_t kk = n - 1 - k;
-      const cpx<R> z = aux[kk] * L[wp((uint32_t)(kk))];
-      return R(2) * z.re;
+      // real line w is part w & 1 of work line w >> 1: its transform from
+      // Z_q and conj Z_{n-q} (q = k, or n - 1 - k for the DST), then
+      // y = 2 Re(w_4n^q V_q)
+      const cpx<R>* Z = W + (w >> 1) * a.Mp;
+      const int64_t q = a.kind == L_DST2 ? n - 1 - k : k;
+      const cpx<R> Zq = Z[wp((uint32_t)q)];
+      const cpx<R> Zn = conj(Z[wp((uint32_t)(q == 0 ? 0 : n - q))]);
+      const cpx<R> V = (w & 1) ? scl(mul_mi(Zq - Zn), R(0.5)) : scl(Zq + Zn, R(0.5));
+      return R(2) * (aux[q] * V).re;
     }
     case L_DCT3:
     case L_DST3: {
       const int64_t m = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-      const R y = L[wp((uint32_t)(m))].re;
+      const cpx<R> z = (W + (w >> 1) * a.Mp)[wp((uint32_t)m)];
+      const R y = (w & 1) ? z.im : z.re;
       return (a.kind == L_DST3 && (k & 1)) ? -y : y;
     }
     case L_DCT4:
