@@ -86,7 +86,7 @@ struct PassArgs {
   int64_t raw_pitch;   // LDS line pitch (scalars) of the raw input tile
   int64_t lds_w1_off;  // byte offset of work buffer 1 (== raw tile) in LDS
   int32_t load_w0;     // c2c, even r2c / c2r: the load writes the work lines (no raw tile)
-  int32_t pair_real;   // DCT/DST I-III: two real lines per complex work line (re, im)
+  int32_t pair_real;   // DCT/DST I-III, DHT: two real lines per complex work line (re, im)
   double scale;        // output multiplier (1 unless ldiv! is fused)
   int32_t do_scale;
   int32_t adjoint;     // fuse the Hermitian weights of the r2c/c2r adjoints

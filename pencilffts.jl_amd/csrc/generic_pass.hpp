@@ -297,9 +297,15 @@ PFFT_HD void phase_pre(const PassArgs& a, const TileCtx& c, int tid, int nthr,
         v = mk(raw_real(a, raw, w, 2 * j), raw_real(a, raw, w, 2 * j + 1));
         break;
       case L_R2C_FULL:
-      case L_DHT:
         v = mk(raw_real(a, raw, w, j), zero);
         break;
+      case L_DHT: {
+        // real lines 2w and 2w + 1 (paired; split after the FFT)
+        const uint32_t nreal = a.r2r_cplx ? 2 * (uint32_t)c.Tact : (uint32_t)c.Tact;
+        v = mk(raw_real(a, raw, 2 * w, j),
+               2 * w + 1 < nreal ? raw_real(a, raw, 2 * w + 1, j) : zero);
+        break;
+      }
       case L_C2R_HALF: {
         cpx<R> A = raw_cplx(a, raw, w, j);
         cpx<R> B = conj(raw_cplx(a, raw, w, M - j));
@@ -788,7 +794,15 @@ PFFT_HD R post_real(const PassArgs& a, const cpx<R>* W, uint32_t w, int64_t k) {
       const cpx<R> z = (W + (w >> 1) * a.Mp)[wp((uint32_t)(k + 1))];
       return (w & 1) ? z.re : -z.im;
     }
-    case L_DHT: return L[wp((uint32_t)(k))].re - L[wp((uint32_t)(k))].im;
+    case L_DHT: {
+      // the transform of real line w (part w & 1 of work line w >> 1) by the
+      // Hermitian split, then Re - Im
+      const cpx<R>* Z = W + (w >> 1) * a.Mp;
+      const cpx<R> Zk = Z[wp((uint32_t)k)];
+      const cpx<R> Zn = conj(Z[wp((uint32_t)(k == 0 ? 0 : n - k))]);
+      const cpx<R> V = (w & 1) ? scl(mul_mi(Zk - Zn), R(0.5)) : scl(Zk + Zn, R(0.5));
+      return V.re - V.im;
+    }
     default: return R(0);
   }
 }

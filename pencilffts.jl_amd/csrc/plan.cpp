@@ -533,11 +533,11 @@ static int line_layout(Plan& p, PassPlan& pp, std::vector<LineDim> lds, int dim,
   // even c2r) load straight into the work lines: no raw tile, no pre phase
   a.load_w0 = (lk == L_C2C || lk == L_R2C_HALF || lk == L_C2R_HALF) ? 1 : 0;
   // Two real R2R lines share one complex FFT as its real and imaginary
-  // parts: DCT-I / DST-I (real transforms), DCT-II / DST-II (split by
-  // Hermitian symmetry afterwards) and DCT-III / DST-III (Hermitian input,
-  // real transforms)
+  // parts: DCT-I / DST-I (real transforms), DCT-II / DST-II and DHT (split
+  // by Hermitian symmetry afterwards) and DCT-III / DST-III (Hermitian
+  // input, real transforms)
   a.pair_real = (lk == L_DCT1 || lk == L_DST1 || lk == L_DCT2 || lk == L_DST2 ||
-                 lk == L_DCT3 || lk == L_DST3) ? 1 : 0;
+                 lk == L_DCT3 || lk == L_DST3 || lk == L_DHT) ? 1 : 0;
   auto footprint = [&](int64_t T, int64_t* w1off) {
     const int64_t nreal = a.r2r_cplx ? 2 * T : T;
     const int64_t Tw = a.pair_real ? (nreal + 1) / 2 : nreal;
