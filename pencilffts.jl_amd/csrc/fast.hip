@@ -30,6 +30,7 @@
 #endif
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <string>
 
@@ -2268,6 +2269,17 @@ std::vector<int> fast_candidates(const Plan& p, const PassPlan& pp, bool op) {
       continue;
     }
     out.push_back((int)i + 1);
+  }
+  // The first candidate is what an untuned plan (PFFT_FLAG_ESTIMATE,
+  // PFFT_FLAG_NO_AUTOTUNE) runs: the cross-lane forms, then one line per
+  // wavefront, which the tuner picks on most FP32 passes (the precision
+  // forms keep their order)
+  if (want_mf == 0) {
+    auto pref = [&](int v) {
+      const Entry& e = reg[variant_index(v)];
+      return (e.mf == 4 ? 0 : 2) + (e.lpw == 1 ? 0 : 1);
+    };
+    std::stable_sort(out.begin(), out.end(), [&](int x, int y) { return pref(x) < pref(y); });
   }
   return out;
 }
