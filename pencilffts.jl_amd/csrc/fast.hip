@@ -1978,6 +1978,11 @@ static void add_2048(std::vector<Entry>& v) {
   add<float, 2048, 4, 1, K_C2C, IO_LINE, IO_TILE, 0, 2, 2>(v);
   add<float, 2048, 4, 1, K_C2C, IO_TILE, IO_LINE>(v);
   add<float, 2048, 4, 1, K_C2C, IO_TILE, IO_LINE, 0, 1, 2>(v);
+  // the r2c / c2r of 4096 reals
+  add<float, 2048, 4, 1, K_R2C, IO_LINE, IO_LINE>(v);
+  add<float, 2048, 4, 1, K_C2R, IO_LINE, IO_LINE>(v);
+  add<float, 2048, 4, 1, K_R2C, IO_LINE, IO_TILE>(v);
+  add<float, 2048, 4, 1, K_C2R, IO_TILE, IO_LINE>(v);
 }
 
 // Float64 1024-point c2c lines with a transposed side: 4-line tiles (64-byte
